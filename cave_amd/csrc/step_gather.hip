@@ -1,0 +1,194 @@
+// step_gather.hip -- cell init, the reference-form step kernel and the
+// per-tick statistics reduction of the single-grid step path.
+//
+// The gather kernel keeps the reference's work decomposition (one lane per
+// packed word = 8 cells, searchSimulationCube.slang:117-216): it is the
+// simplest correct form and the kernel every faster one is checked against on
+// the device.  Each lane loads the 27 words around its own once and builds an
+// 8-bit live mask per word, so the per-cell neighbour test is a shift, not a
+// global load.
+#include "step_kernels.h"
+
+namespace cave {
+
+// ---------------------------------------------------------------------------
+// init: writes the spawn box into a row-pitched nibble layout
+// (searchSystem.cpp:264-303; pitch >= ceil(X/8) words, padding words zero).
+// ---------------------------------------------------------------------------
+__global__ void k_init_cells(uint32_t *cells, GridGeom g, DevInit init, uint32_t live_nibble)
+{
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < n;
+         w += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t xw = (uint32_t)(w % g.pitch);
+        const size_t row = w / g.pitch;
+        const uint32_t y = (uint32_t)(row % g.Y);
+        const uint32_t z = (uint32_t)(row / g.Y);
+        uint32_t out = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 8; i++) {
+            const uint32_t x = xw * 8u + i;
+            if (x < g.X && init_alive(init, x, y, z)) out |= live_nibble << (4u * i);
+        }
+        cells[w] = out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gather step: one lane per packed word
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t live_mask8(uint32_t w, uint32_t M)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) m |= (((w >> (4u * i)) & 15u) == M ? 1u : 0u) << i;
+    return m;
+}
+
+__global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                     GridGeom g, DevRule rule, StatPart *__restrict__ partials)
+{
+    __shared__ int32_t red[10 * 4];
+    StatAcc acc;
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    const size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const uint32_t upr = (g.X + 7u) / 8u;
+    if (w < n) {
+        const uint32_t xw = (uint32_t)(w % g.pitch);
+        const size_t row = w / g.pitch;
+        const int y = (int)(row % g.Y);
+        const int z = (int)(row / g.Y);
+        if (xw < upr) {
+            // 24-bit live masks of the 9 rows around (y, z): [word-1 | word | word+1]
+            uint32_t rm[3][3];
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int yy = y + dy, zz = z + dz;
+                    uint32_t m = 0;
+                    if (yy >= 0 && yy < (int)g.Y && zz >= 0 && zz < (int)g.Z) {
+                        const size_t base = ((size_t)zz * g.Y + yy) * g.pitch;
+                        const uint32_t wl = xw > 0 ? in[base + xw - 1] : 0u;
+                        const uint32_t wc = in[base + xw];
+                        const uint32_t wr = xw + 1 < upr ? in[base + xw + 1] : 0u;
+                        m = live_mask8(wl, rule.M) | (live_mask8(wc, rule.M) << 8) | (live_mask8(wr, rule.M) << 16);
+                    }
+                    rm[dz + 1][dy + 1] = m;
+                }
+            const uint32_t cur_word = in[w];
+            uint32_t out_word = 0;
+            const bool fF = rule.flags & 1u, fE = rule.flags & 2u, fC = rule.flags & 4u;
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) {
+                const uint32_t x = xw * 8u + i;
+                if (x >= g.X) break;
+                uint32_t nb = 0;
+#pragma unroll
+                for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
+                            const bool en = cls == 1 ? fF : cls == 2 ? fE : cls == 3 ? fC : false;
+                            if (en) nb += (rm[dz + 1][dy + 1] >> (8 + (int)i + dx)) & 1u;
+                        }
+                const uint32_t cur = (cur_word >> (4u * i)) & 15u;
+                uint32_t ns;
+                // searchSimulationCube.slang:178-211
+                if (cur == rule.M) {
+                    if ((rule.surv_lut >> nb) & 1u) { ns = rule.M; acc.surv++; }
+                    else { ns = cur - 1u; acc.dying++; }
+                } else if (cur == 0u) {
+                    if ((rule.born_lut >> nb) & 1u) { ns = rule.M; acc.born++; }
+                    else { ns = 0u; acc.dead++; }
+                } else {
+                    ns = cur - 1u;
+                    acc.dying++;
+                }
+                out_word |= (ns & 15u) << (4u * i);
+                if (ns == rule.M) {
+                    acc.bmin[0] = min(acc.bmin[0], (int32_t)x); acc.bmax[0] = max(acc.bmax[0], (int32_t)x);
+                    acc.bmin[1] = min(acc.bmin[1], (int32_t)y); acc.bmax[1] = max(acc.bmax[1], (int32_t)y);
+                    acc.bmin[2] = min(acc.bmin[2], (int32_t)z); acc.bmax[2] = max(acc.bmax[2], (int32_t)z);
+                }
+            }
+            out[w] = out_word;
+        } else {
+            out[w] = 0u;
+        }
+    }
+    block_reduce_stats(acc, &partials[blockIdx.x], red);
+}
+
+// ---------------------------------------------------------------------------
+// per-tick reduction of block partials: one block per tick
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_reduce_ticks(const StatPart *__restrict__ partials, uint32_t nparts,
+                                                      CaveTickStats *__restrict__ out)
+{
+    __shared__ int32_t red[10 * 4];
+    const StatPart *p = partials + (size_t)blockIdx.x * nparts;
+    StatAcc acc;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const StatPart q = p[i];
+        acc.dead += q.dead; acc.dying += q.dying; acc.born += q.born; acc.surv += q.surv;
+        for (int k = 0; k < 3; k++) {
+            acc.bmin[k] = min(acc.bmin[k], q.bmin[k]);
+            acc.bmax[k] = max(acc.bmax[k], q.bmax[k]);
+        }
+    }
+    __shared__ StatPart tot;
+    block_reduce_stats(acc, &tot, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CaveTickStats s;
+        s.snap.numberOfDeadCells = tot.dead;
+        s.snap.numberOfDyingCells = tot.dying;
+        s.snap.numberOfCellsBorn = tot.born;
+        s.snap.numberOfSurvivingCells = tot.surv;
+        s.live = tot.born + tot.surv;
+        s.changed = tot.born + tot.dying;
+        for (int k = 0; k < 3; k++) {
+            // empty box reported as min = 0x7fffffff, max = -1
+            s.bbox_min[k] = tot.bmin[k];
+            s.bbox_max[k] = tot.bmax[k];
+        }
+        out[blockIdx.x] = s;
+    }
+}
+
+int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st)
+{
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 65535u * 4u);
+    hipLaunchKernelGGL(k_init_cells, dim3(blocks), dim3(256), 0, st, cells, g, init, M & 15u);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+uint32_t gather_blocks(const GridGeom &g)
+{
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    return (uint32_t)((n + 255) / 256);
+}
+
+int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_gather, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
+                        hipStream_t st)
+{
+    if (ticks == 0) return CAVE_OK;
+    hipLaunchKernelGGL(k_reduce_ticks, dim3(ticks), dim3(256), 0, st, partials, nparts, out);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+}  // namespace cave

@@ -1,0 +1,47 @@
+// step_kernels.h -- launch interface of the single-grid step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "common.h"
+#include "device_util.h"
+
+namespace cave {
+
+// Row-pitched nibble layout: cell (x, y, z) lives in word x/8 + pitch*(y + Y*z),
+// nibble x%8.  pitch = ceil(X/8) is the reference layout; the streaming kernel
+// pads rows to whole 32-cell groups (pitch = 4*ceil(X/32)).
+struct GridGeom {
+    uint32_t X, Y, Z;
+    uint32_t pitch;
+};
+
+int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st);
+
+uint32_t gather_blocks(const GridGeom &g);
+int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
+                  hipStream_t st);
+
+int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
+                        hipStream_t st);
+
+// ---- bit-sliced streaming kernel ----
+struct StreamPlan {
+    uint32_t lpr;      // lanes per x-row (power of two, 2..16)
+    uint32_t rpl;      // rows per lane (y register blocking)
+    uint32_t tz;       // z planes per wave
+    uint32_t waves;    // waves per block
+    uint32_t blocks;   // grid size
+    uint32_t ytiles;   // y tiles of (64/lpr)*rpl rows
+    uint32_t ztiles;
+};
+
+bool stream_supported(const GridGeom &g);
+StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t waves);
+int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, const StreamPlan &p,
+                  StatPart *partials, hipStream_t st);
+
+}  // namespace cave

@@ -1,0 +1,294 @@
+/*
+ * cave.h -- C-ABI of the MI355X-native CAVE step + rule-search engine.
+ *
+ * Plain C linkage, plain pointers and sizes.  Host buffers are caller-owned;
+ * device memory is library-owned.  Every entry point returns an int status
+ * (CAVE_OK = 0) and never throws; cave_last_error() gives the message of the
+ * calling thread's last failure.  A handle is used by one thread at a time;
+ * different handles may be used from different threads.
+ *
+ * Reference interfaces replaced (paths relative to the CAVE repository):
+ *   CaveSimulationParameters  <- SimulationParameters   src/common/structs.h:89-99
+ *   CaveGridSnapshot          <- GridSnapshot           src/common/structs.h:130-136
+ *   CaveGridInfo              <- GridInfo               src/common/structs.h:79-87
+ *   CaveChunkConfiguration    <- ChunkConfiguration     src/common/structs.h:109-128
+ *   cave_step_*               <- ComputeSystem::Tick + the headless step loop
+ *                                src/vulkan/systems/computeSystem.h:54-56,
+ *                                src/modes/renderingMode.cpp:166-343, with the
+ *                                search-kernel transition of
+ *                                shaders/slang/search/searchSimulationCube.slang:78-247
+ *   cave_search_*             <- SearchSystem (ctor, IsTickComplete/FlushProgress/
+ *                                SubmitNextTick, IsRunning, GetProgress,
+ *                                Get*Results, WriteToDisk)
+ *                                src/vulkan/systems/searchSystem.h:218-275
+ *   cave_rule_string_to_bitmask / cave_bitmask_to_rule_string
+ *                             <- GuiSystem::RuleStringToBitmask / BitmaskToRuleString
+ *                                src/vulkan/systems/guiSystem.cpp:92-163
+ *   cave_enumerate_valid_bitmasks
+ *                             <- PermutationMath::EnumerateValidBitmasks
+ *                                src/common/permutationMath.h:101-128
+ *   cave_batch_*, cave_serve_* have no reference counterpart: batched
+ *   evaluation of explicit (rule, C, grid, seed, ticks) lists and a
+ *   continuous-batching request server.
+ */
+#ifndef CAVE_H
+#define CAVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAVE_ABI_VERSION 1
+
+/* ---- status codes ---- */
+enum {
+    CAVE_OK = 0,
+    CAVE_E_INVALID = 1,     /* bad argument / spec                      */
+    CAVE_E_UNSUPPORTED = 2, /* valid in the reference, not built here   */
+    CAVE_E_NOMEM = 3,       /* device or host allocation failed         */
+    CAVE_E_DEVICE = 4,      /* HIP runtime error                        */
+    CAVE_E_BUDGET = 5,      /* request larger than the admission budget */
+    CAVE_E_STATE = 6        /* call not valid in the handle's state     */
+};
+
+/* ---- neighbourhood flags (src/simulation/simulation.h:47-50) ---- */
+#define CAVE_FACES 1u
+#define CAVE_EDGES 2u
+#define CAVE_CORNERS 4u
+#define CAVE_WRAP 8u
+
+/* ---- reference structs, byte-identical ---- */
+typedef struct CaveSimulationParameters {
+    uint64_t birthAndMaxCellStateRules;    /* bits 60-63: M-1; bits 0-59: birth mask or 1-based index */
+    uint64_t survivalAndNeighborhoodRules; /* bits 60-63: F|E|C|W flags; bits 0-59: survival mask or index */
+    uint64_t shapeAndGridConfiguration;    /* bits 0-4: shape id (0 = cube) */
+} CaveSimulationParameters;
+
+typedef struct CaveGridSnapshot {
+    uint32_t numberOfDeadCells;      /* 0 -> 0                 */
+    uint32_t numberOfDyingCells;     /* M -> M-1 or 0<s<M -> s-1 */
+    uint32_t numberOfCellsBorn;      /* 0 -> M                 */
+    uint32_t numberOfSurvivingCells; /* M -> M                 */
+} CaveGridSnapshot;
+
+typedef struct CaveGridInfo {
+    uint32_t totalNumberOfCellsThatDied;
+    uint32_t totalNumberOfCellsThatWereBorn;
+    uint32_t numberOfSimulationTicksSurvived;
+    uint32_t status; /* 0 running, 1 died, 2 reached the tick budget */
+    uint32_t maxAliveCount;
+    uint32_t finalAliveCount;
+} CaveGridInfo;
+
+typedef struct CaveChunkConfiguration {
+    uint32_t shapeId;
+    uint32_t gridDimensionX, gridDimensionY, gridDimensionZ;
+    uint32_t spawnAreaDimensionX, spawnAreaDimensionY, spawnAreaDimensionZ;
+    uint32_t spawnPlacementId;
+    uint32_t spawnRegionShapeId; /* 0 = random, 1 = filled */
+    uint32_t spawnSizeIndex;
+    uint32_t spawnDensityIndex;  /* density = (index + 1) / 16 */
+    uint32_t runSeedIndex;
+    uint32_t partitionIndex;
+    uint32_t partitionCount;
+} CaveChunkConfiguration;
+
+/* Per-tick statistics: the reference counters plus live / changed counts
+ * and the bounding box of live (state == M) cells after the tick.
+ * bbox_min[a] > bbox_max[a] means no live cell. */
+typedef struct CaveTickStats {
+    CaveGridSnapshot snap;
+    uint32_t live;    /* born + surviving */
+    uint32_t changed; /* born + dying     */
+    int32_t bbox_min[3];
+    int32_t bbox_max[3];
+} CaveTickStats;
+
+/* ---- library ---- */
+int cave_abi_version(void);
+const char *cave_last_error(void);
+/* name of device 0, its CU count and HBM bytes */
+int cave_device_info(char *name, size_t cap, int *cu_count, uint64_t *hbm_bytes);
+
+/* ---- host helpers (no device work) ---- */
+uint64_t cave_rule_string_to_bitmask(const char *rule);
+size_t cave_bitmask_to_rule_string(uint64_t bitmask, char *out, size_t cap);
+size_t cave_enumerate_valid_bitmasks(uint32_t neighbor_count, uint32_t max_bits, uint64_t *out, size_t cap);
+/* words of the reference packed layout: ceil(X/8) * Y * Z */
+size_t cave_cell_words(uint32_t X, uint32_t Y, uint32_t Z);
+/* splitmix64 generator used for every synthetic workload (seeded, portable) */
+uint64_t cave_splitmix64(uint64_t *state);
+/* random rule bitset over counts 1..n_counts, each bit set with probability p */
+uint64_t cave_random_rule(uint64_t *state, uint32_t n_counts, double p);
+
+/* ================================================================== */
+/* Single-grid step (headless step path)                               */
+/* ================================================================== */
+typedef struct CaveStep CaveStep;
+
+enum {
+    CAVE_SPAWN_RANDOM = 0, /* hash(x,y,z,seed) < floor(density * 2^32) inside the box */
+    CAVE_SPAWN_FILLED = 1, /* every box cell live */
+    CAVE_SPAWN_BUFFER = 2  /* caller-supplied cells (reference packed layout) */
+};
+
+enum {
+    CAVE_KERNEL_AUTO = 0,
+    CAVE_KERNEL_GATHER = 1, /* one lane per packed word, per-cell gather */
+    CAVE_KERNEL_STREAM = 2  /* bit-sliced z-streaming kernel            */
+};
+
+typedef struct CaveStepSpec {
+    uint32_t dims[3];
+    CaveSimulationParameters params; /* raw-bitmask form (max-rule-bits 0); W is rejected */
+    uint32_t spawn[3];               /* spawn box, centred, clamped to dims */
+    uint32_t spawn_mode;             /* CAVE_SPAWN_* */
+    double density;                  /* random spawn probability, evaluated in double */
+    uint32_t seed;                   /* runSeedIndex salt of the cell hash */
+    const uint32_t *initial_cells;   /* CAVE_SPAWN_BUFFER: cave_cell_words(dims) words */
+    uint32_t kernel;                 /* CAVE_KERNEL_* */
+    uint32_t reserved;
+} CaveStepSpec;
+
+int cave_step_create(const CaveStepSpec *spec, CaveStep **out);
+/* Runs `ticks` ticks.  per_tick (nullable) receives `ticks` entries.
+ * device_ms (nullable) receives the device time of the tick loop, measured
+ * with HIP events on the handle's stream. */
+int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *device_ms);
+/* Copies the current cells out in the reference packed layout. */
+int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words);
+int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words);
+/* Replaces the rule (same M / flags constraints as create). */
+int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params);
+/* Which kernel the handle runs (CAVE_KERNEL_GATHER or CAVE_KERNEL_STREAM). */
+int cave_step_kernel(const CaveStep *h);
+int cave_step_destroy(CaveStep *h);
+
+/* ================================================================== */
+/* Batched evaluation of explicit configs                              */
+/* ================================================================== */
+typedef struct CaveSimSpec {
+    uint32_t dims[3];
+    uint32_t max_cell_state;     /* M, 1..16 (C = M + 1 states) */
+    uint32_t neighborhood_flags; /* F|E|C; W rejected */
+    uint32_t ticks;              /* tick budget (MAX_NUMBER_OF_TICKS_TO_SURVIVE) */
+    uint64_t birth;              /* raw bitmask, bit b = count b+1 */
+    uint64_t survival;
+    uint32_t spawn[3];
+    uint32_t spawn_mode; /* CAVE_SPAWN_RANDOM or CAVE_SPAWN_FILLED */
+    double density;
+    uint32_t seed;
+    uint32_t flags; /* CAVE_SIM_* */
+} CaveSimSpec;
+
+#define CAVE_SIM_NO_EARLY_EXIT 1u /* run all ticks even after the manager's status 1 */
+#define CAVE_SIM_FINAL_HASH 2u    /* compute the FNV-1a canonical hash of the final cells */
+
+typedef struct CaveSimResult {
+    CaveGridInfo info;       /* GridInfo at completion (manager semantics) */
+    uint32_t classification; /* 2 viable, 1 possible, 0 unviable */
+    uint32_t ticks_run;
+    CaveTickStats last;      /* stats of the last tick run */
+    uint64_t final_hash;     /* with CAVE_SIM_FINAL_HASH */
+} CaveSimResult;
+
+enum {
+    CAVE_ENGINE_AUTO = 0,
+    CAVE_ENGINE_PERSISTENT = 1, /* one persistent work-stealing launch for the batch */
+    CAVE_ENGINE_PLAIN = 2       /* one step launch per config per tick */
+};
+
+typedef struct CaveBatchOptions {
+    uint32_t engine;         /* CAVE_ENGINE_* */
+    uint32_t block_threads;  /* 0 = autotuned */
+    CaveTickStats *per_tick; /* nullable: n_specs * max(ticks) entries, row per spec */
+    uint32_t per_tick_stride; /* entries per spec row in per_tick (>= max ticks) */
+    uint32_t reserved;
+    double device_ms;        /* out: device time of the evaluation */
+} CaveBatchOptions;
+
+int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *results,
+                   CaveBatchOptions *opt);
+
+/* ================================================================== */
+/* Reference-compatible --mode search chunk (SearchSystem equivalent)  */
+/* ================================================================== */
+typedef struct CaveSearch CaveSearch;
+
+typedef struct CaveSearchSpec {
+    CaveChunkConfiguration chunk;
+    uint32_t neighborhood_flags; /* selected F|E|C; every non-empty subset is a slot */
+    uint32_t max_rule_bits;      /* K (0 = raw bitmask space) */
+    int32_t min_max_cell_state;  /* --min-cs */
+    int32_t max_max_cell_state;  /* --max-cs-range */
+    uint32_t max_ticks;          /* --ticks */
+    uint32_t reserved;
+    const char *output_folder;   /* nullable: no JSONL files */
+    const char *file_prefix;     /* "<prefix>_{viable,possible,unviable}.jsonl" */
+} CaveSearchSpec;
+
+/* SearchResult (src/vulkan/systems/searchSystem.h), rule words in bitmask form */
+typedef struct CaveSearchResult {
+    CaveSimulationParameters permutation;
+    uint32_t ticksSurvived;
+    uint32_t totalBirths;
+    uint32_t totalDeaths;
+    uint32_t maxAliveCount;
+    uint32_t finalAliveCount;
+    uint32_t category; /* 2 viable, 1 possible, 0 unviable */
+} CaveSearchResult;
+
+typedef struct CaveSearchProgress {
+    uint64_t total;      /* rule evaluations in this chunk */
+    uint64_t completed;
+    uint64_t viable, possible, unviable;
+} CaveSearchProgress;
+
+int cave_search_create(const CaveSearchSpec *spec, CaveSearch **out);
+/* Evaluates up to max_evals pending rules (0 = all) in batched launches. */
+int cave_search_step(CaveSearch *h, uint64_t max_evals);
+int cave_search_is_running(const CaveSearch *h);
+int cave_search_progress(const CaveSearch *h, CaveSearchProgress *out);
+/* Moves up to cap finished results out; *n receives the count. */
+int cave_search_drain(CaveSearch *h, CaveSearchResult *out, size_t cap, size_t *n);
+/* Flushes JSONL files; writes the reference summary line into msg. */
+int cave_search_write_summary(CaveSearch *h, char *msg, size_t cap);
+int cave_search_destroy(CaveSearch *h);
+
+/* ================================================================== */
+/* Continuous-batching request server                                  */
+/* ================================================================== */
+typedef struct CaveServer CaveServer;
+
+typedef struct CaveServeOptions {
+    uint64_t hbm_budget_bytes; /* admission budget for resident state */
+    uint32_t max_slots;        /* concurrent resident requests (0 = auto) */
+    uint32_t slice_ticks;      /* ticks per scheduling round (0 = auto) */
+} CaveServeOptions;
+
+typedef struct CaveServeCompletion {
+    uint64_t request_id;
+    CaveSimResult result;
+    double submit_s;  /* host clock, seconds since server start */
+    double admit_s;
+    double finish_s;
+} CaveServeCompletion;
+
+int cave_serve_create(const CaveServeOptions *opt, CaveServer **out);
+/* Queues a request; CAVE_E_BUDGET if it can never fit the budget. */
+int cave_serve_submit(CaveServer *s, const CaveSimSpec *spec, uint64_t request_id);
+/* Moves finished requests out (non-blocking). */
+int cave_serve_poll(CaveServer *s, CaveServeCompletion *out, size_t cap, size_t *n);
+/* Blocks until every submitted request finished. */
+int cave_serve_drain(CaveServer *s);
+int cave_serve_stats(const CaveServer *s, uint64_t *admitted, uint64_t *finished,
+                     uint64_t *resident_bytes, uint64_t *peak_resident_bytes);
+int cave_serve_destroy(CaveServer *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""ctypes binding of the CPU oracle (oracle/cave_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product (cave_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libcave_oracle.so")
+
+
+class GridInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "totalNumberOfCellsThatDied", "totalNumberOfCellsThatWereBorn",
+        "numberOfSimulationTicksSurvived", "status", "maxAliveCount",
+        "finalAliveCount")]
+
+
+class Snapshot(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "numberOfDeadCells", "numberOfDyingCells", "numberOfCellsBorn",
+        "numberOfSurvivingCells")]
+
+
+class Permutation(C.Structure):
+    _fields_ = [("birthAndMaxCellStateRules", C.c_uint64),
+                ("survivalAndNeighborhoodRules", C.c_uint64)]
+
+
+class TickStats(C.Structure):
+    _fields_ = [("snap", Snapshot), ("live", C.c_uint32), ("changed", C.c_uint32),
+                ("bbox_min", C.c_int32 * 3), ("bbox_max", C.c_int32 * 3)]
+
+    def as_tuple(self):
+        s = self.snap
+        return (s.numberOfDeadCells, s.numberOfDyingCells, s.numberOfCellsBorn,
+                s.numberOfSurvivingCells, self.live, self.changed,
+                tuple(self.bbox_min), tuple(self.bbox_max))
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        u32, u64, p32 = C.c_uint32, C.c_uint64, C.POINTER(C.c_uint32)
+        L.orc_words_per_row.restype = u32
+        L.orc_cell_words.restype = C.c_size_t
+        L.orc_cell_words.argtypes = [u32, u32, u32]
+        L.orc_canonical_hash.restype = u64
+        L.orc_canonical_hash.argtypes = [p32, u32, u32, u32]
+        L.orc_wang_hash.restype = u32
+        L.orc_wang_hash.argtypes = [u32]
+        L.orc_cell_hash.restype = u32
+        L.orc_cell_hash.argtypes = [u32, u32, u32, u32]
+        L.orc_density_threshold.restype = u32
+        L.orc_density_threshold.argtypes = [C.c_double]
+        L.orc_init_cells.restype = None
+        L.orc_init_cells.argtypes = [p32, u32, u32, u32, u32, u32, u32, u32, u32, C.c_double, u32]
+        L.orc_step.restype = None
+        L.orc_step.argtypes = [p32, p32, u32, u32, u32, u64, u64, C.POINTER(C.c_uint64), u32,
+                               C.POINTER(Snapshot)]
+        L.orc_step_stats.restype = None
+        L.orc_step_stats.argtypes = [p32, p32, u32, u32, u32, u64, u64, C.POINTER(C.c_uint64), u32,
+                                     C.POINTER(TickStats)]
+        L.orc_manager_tick.restype = u32
+        L.orc_manager_tick.argtypes = [C.POINTER(GridInfo), C.POINTER(Snapshot), C.POINTER(Permutation),
+                                       u64, u64, u32, C.POINTER(GridInfo), C.POINTER(Permutation)]
+        L.orc_classify.restype = u32
+        L.orc_classify.argtypes = [C.POINTER(GridInfo), u32]
+        L.orc_enumerate_valid_bitmasks.restype = C.c_size_t
+        L.orc_enumerate_valid_bitmasks.argtypes = [u32, u32, C.POINTER(C.c_uint64), C.c_size_t]
+        L.orc_rule_permutation_count.restype = u64
+        L.orc_rule_permutation_count.argtypes = [u32]
+        L.orc_neighbor_count.restype = u32
+        L.orc_neighbor_count.argtypes = [u32]
+        L.orc_search_total_permutations.restype = u64
+        L.orc_search_total_permutations.argtypes = [C.c_int] * 6
+        L.orc_rule_string_to_bitmask.restype = u64
+        L.orc_rule_string_to_bitmask.argtypes = [C.c_char_p]
+        L.orc_bitmask_to_rule_string.restype = C.c_size_t
+        L.orc_bitmask_to_rule_string.argtypes = [u64, C.c_char_p, C.c_size_t]
+        L.orc_run_rule.restype = u32
+        L.orc_run_rule.argtypes = [u32, u32, u32, u32, u32, u64, u64, u32, p32, C.POINTER(GridInfo),
+                                   C.POINTER(TickStats), p32]
+        L.orc_run_search.restype = u64
+        L.orc_run_search.argtypes = [u32, u32, u32, C.c_int, C.c_int, u32, u32, u32, C.c_double, u32,
+                                     C.c_int, C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def _p32(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def cell_words(X, Y, Z) -> int:
+    return int(lib().orc_cell_words(X, Y, Z))
+
+
+def init_cells(X, Y, Z, M, spawn, spawn_mode, density, seed) -> np.ndarray:
+    a = np.zeros(cell_words(X, Y, Z), dtype=np.uint32)
+    sx, sy, sz = spawn
+    lib().orc_init_cells(_p32(a), X, Y, Z, M, sx, sy, sz, spawn_mode, float(density), seed)
+    return a
+
+
+def step(cells, X, Y, Z, word0, word1, valid_bitmasks=None, max_rule_bits=0):
+    out = np.zeros_like(cells)
+    st = TickStats()
+    vb = None
+    if valid_bitmasks is not None:
+        vb_arr = np.ascontiguousarray(valid_bitmasks, dtype=np.uint64)
+        vb = vb_arr.ctypes.data_as(C.POINTER(C.c_uint64))
+    lib().orc_step_stats(_p32(cells), _p32(out), X, Y, Z, word0, word1, vb, max_rule_bits,
+                         C.byref(st))
+    return out, st
+
+
+def canonical_hash(cells, X, Y, Z) -> int:
+    return int(lib().orc_canonical_hash(_p32(cells), X, Y, Z))
+
+
+def enumerate_valid_bitmasks(N, K):
+    n = lib().orc_enumerate_valid_bitmasks(N, K, None, 0)
+    out = (C.c_uint64 * max(n, 1))()
+    lib().orc_enumerate_valid_bitmasks(N, K, out, n)
+    return [int(out[i]) for i in range(n)]
+
+
+def rule_string_to_bitmask(s: str) -> int:
+    return int(lib().orc_rule_string_to_bitmask(s.encode()))
+
+
+def bitmask_to_rule_string(m: int) -> str:
+    buf = C.create_string_buffer(512)
+    lib().orc_bitmask_to_rule_string(m, buf, 512)
+    return buf.value.decode()
+
+
+def run_rule(X, Y, Z, M, flags, birth, survival, max_ticks, initial):
+    info = GridInfo()
+    per_tick = (TickStats * max_ticks)()
+    final = np.zeros_like(initial)
+    t = lib().orc_run_rule(X, Y, Z, M, flags, birth, survival, max_ticks, _p32(initial),
+                           C.byref(info), per_tick, _p32(final))
+    return int(t), info, [per_tick[i] for i in range(min(t, max_ticks))], final
+
+
+def run_search(grid, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn, spawn_mode,
+               density, seed, n_threads=8):
+    counts = (C.c_uint64 * 3)()
+    n = lib().orc_run_search(grid, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn,
+                             spawn_mode, float(density), seed, n_threads, counts)
+    return int(n), (int(counts[0]), int(counts[1]), int(counts[2]))

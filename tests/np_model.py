@@ -1,0 +1,64 @@
+"""Independent numpy formulation of the CAVE cube step, used to pin the oracle.
+
+It is written from the mathematics, not from the reference's code: the
+neighbourhood is {d in {-1,0,1}^3 : |d|_1 in selected classes} (class 1 =
+faces, 2 = edges, 3 = corners), counted by slicing a zero-padded live array,
+and the transition is the three-case table of the search kernel
+(searchSimulationCube.slang:178-211).  Cells are a dense (Z, Y, X) uint8
+array; pack/unpack convert to the reference row-padded nibble layout.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+
+def unpack(words: np.ndarray, X: int, Y: int, Z: int) -> np.ndarray:
+    upr = (X + 7) // 8
+    w = np.asarray(words, dtype=np.uint32).reshape(Z, Y, upr)
+    nib = np.stack([(w >> (4 * i)) & 0xF for i in range(8)], axis=-1).reshape(Z, Y, upr * 8)
+    return nib[:, :, :X].astype(np.uint8)
+
+
+def pack(cells: np.ndarray) -> np.ndarray:
+    Z, Y, X = cells.shape
+    upr = (X + 7) // 8
+    pad = np.zeros((Z, Y, upr * 8), dtype=np.uint32)
+    pad[:, :, :X] = cells
+    pad = pad.reshape(Z, Y, upr, 8)
+    w = np.zeros((Z, Y, upr), dtype=np.uint32)
+    for i in range(8):
+        w |= pad[..., i] << np.uint32(4 * i)
+    return w.reshape(-1)
+
+
+def offsets(flags: int):
+    for d in itertools.product((-1, 0, 1), repeat=3):
+        c = sum(1 for v in d if v)
+        if c and (flags >> (c - 1)) & 1:
+            yield d
+
+
+def neighbor_counts(live: np.ndarray, flags: int) -> np.ndarray:
+    Z, Y, X = live.shape
+    p = np.zeros((Z + 2, Y + 2, X + 2), dtype=np.int32)
+    p[1:-1, 1:-1, 1:-1] = live
+    n = np.zeros((Z, Y, X), dtype=np.int32)
+    for dx, dy, dz in offsets(flags):
+        n += p[1 + dz:1 + dz + Z, 1 + dy:1 + dy + Y, 1 + dx:1 + dx + X]
+    return n
+
+
+def step(cells: np.ndarray, M: int, flags: int, birth: int, survival: int):
+    """Returns (new cells, (dead, dying, born, surviving))."""
+    live = (cells == M).astype(np.int32)
+    n = neighbor_counts(live, flags)
+    bit = lambda mask: np.where(n > 0, (np.uint64(mask) >> (np.maximum(n, 1) - 1).astype(np.uint64)) & np.uint64(1), 0).astype(bool)
+    surv = (cells == M) & bit(survival)
+    born = (cells == 0) & bit(birth)
+    out = np.where(cells == 0, 0, cells.astype(np.int32) - 1)
+    out = np.where(surv | born, M, out).astype(np.int64) & 0xF
+    dead = int(((cells == 0) & ~born).sum())
+    dying = int(((cells != 0) & ~surv).sum())
+    return out.astype(np.uint8), (dead, dying, int(born.sum()), int(surv.sum()))
