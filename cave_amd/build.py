@@ -33,7 +33,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 LIB_SOURCES = [
     "api_common.cpp",
     "api_step.cpp",
+    "api_batch.cpp",
     "api_pending.cpp",
+    "batch.hip",
     "step_gather.hip",
     "step_stream.hip",
 ]

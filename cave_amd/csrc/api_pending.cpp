@@ -5,11 +5,6 @@ using namespace cave;
 
 extern "C" {
 
-int cave_batch_run(const CaveSimSpec *, size_t, CaveSimResult *, CaveBatchOptions *)
-{
-    return fail(CAVE_E_UNSUPPORTED, "batch engine not built");
-}
-
 int cave_search_create(const CaveSearchSpec *, CaveSearch **out)
 {
     if (out) *out = nullptr;

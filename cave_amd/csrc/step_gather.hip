@@ -46,9 +46,11 @@ __device__ inline uint32_t live_mask8(uint32_t w, uint32_t M)
 }
 
 __global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     GridGeom g, DevRule rule, StatPart *__restrict__ partials)
+                                                     GridGeom g, DevRule rule, StatPart *__restrict__ partials,
+                                                     const uint32_t *__restrict__ done)
 {
     __shared__ int32_t red[10 * 4];
+    if (done && *done) return;  // the manager already ended this simulation
     StatAcc acc;
     const size_t n = (size_t)g.pitch * g.Y * g.Z;
     const size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -159,6 +161,65 @@ __global__ void __launch_bounds__(256) k_reduce_ticks(const StatPart *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// plain per-config manager (one block): reduce the tick's block partials and
+// apply searchManager.slang:115-139 on the device
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_plain_manager(const StatPart *__restrict__ partials, uint32_t nparts,
+                                                       PlainState *__restrict__ s, uint32_t ticks,
+                                                       uint32_t sim_flags, CaveTickStats *__restrict__ per_tick)
+{
+    __shared__ int32_t red[10 * 4];
+    __shared__ StatPart tot;
+    if (s->done) return;
+    StatAcc acc;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const StatPart q = partials[i];
+        acc.dead += q.dead; acc.dying += q.dying; acc.born += q.born; acc.surv += q.surv;
+        for (int k = 0; k < 3; k++) {
+            acc.bmin[k] = min(acc.bmin[k], q.bmin[k]);
+            acc.bmax[k] = max(acc.bmax[k], q.bmax[k]);
+        }
+    }
+    block_reduce_stats(acc, &tot, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CaveGridInfo info = s->info;
+        info.numberOfSimulationTicksSurvived++;
+        info.totalNumberOfCellsThatDied += tot.dead;
+        info.totalNumberOfCellsThatWereBorn += tot.born;
+        info.finalAliveCount = tot.surv;
+        info.maxAliveCount = max(info.maxAliveCount, tot.surv);
+        if (tot.surv == 0u && tot.born == 0u) info.status = 1u;
+        else if (info.numberOfSimulationTicksSurvived >= ticks) info.status = 2u;
+        else info.status = 0u;
+        CaveTickStats l;
+        l.snap.numberOfDeadCells = tot.dead;
+        l.snap.numberOfDyingCells = tot.dying;
+        l.snap.numberOfCellsBorn = tot.born;
+        l.snap.numberOfSurvivingCells = tot.surv;
+        l.live = tot.born + tot.surv;
+        l.changed = tot.born + tot.dying;
+        for (int k = 0; k < 3; k++) {
+            l.bbox_min[k] = tot.bmin[k];
+            l.bbox_max[k] = tot.bmax[k];
+        }
+        if (per_tick) per_tick[info.numberOfSimulationTicksSurvived - 1u] = l;
+        s->info = info;
+        s->last = l;
+        s->done = ((info.status == 1u && !(sim_flags & CAVE_SIM_NO_EARLY_EXIT)) ||
+                   info.numberOfSimulationTicksSurvived >= ticks) ? 1u : 0u;
+    }
+}
+
+int launch_plain_manager(const StatPart *partials, uint32_t nparts, PlainState *state, uint32_t ticks,
+                         uint32_t sim_flags, CaveTickStats *per_tick, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_plain_manager, dim3(1), dim3(256), 0, st, partials, nparts, state, ticks, sim_flags, per_tick);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
 int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st)
 {
     const size_t n = (size_t)g.pitch * g.Y * g.Z;
@@ -175,9 +236,9 @@ uint32_t gather_blocks(const GridGeom &g)
 }
 
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
-                  hipStream_t st)
+                  hipStream_t st, const uint32_t *done)
 {
-    hipLaunchKernelGGL(k_step_gather, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials);
+    hipLaunchKernelGGL(k_step_gather, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials, done);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

@@ -22,8 +22,10 @@ struct GridGeom {
 int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st);
 
 uint32_t gather_blocks(const GridGeom &g);
+// done (nullable): device flag; when set the launch does nothing (the plain
+// per-config driver ends a simulation on the device without a host round trip)
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
-                  hipStream_t st);
+                  hipStream_t st, const uint32_t *done = nullptr);
 
 int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
                         hipStream_t st);
@@ -42,6 +44,17 @@ struct StreamPlan {
 bool stream_supported(const GridGeom &g);
 StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t waves);
 int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, const StreamPlan &p,
-                  StatPart *partials, hipStream_t st);
+                  StatPart *partials, hipStream_t st, const uint32_t *done = nullptr);
+
+// ---- plain per-config manager: reduces one tick's partials and applies the
+// search manager on the device (searchManager.slang:98-149) ----
+struct PlainState {
+    CaveGridInfo info;
+    uint32_t done;
+    uint32_t pad;
+    CaveTickStats last;
+};
+int launch_plain_manager(const StatPart *partials, uint32_t nparts, PlainState *state, uint32_t ticks,
+                         uint32_t sim_flags, CaveTickStats *per_tick, hipStream_t st);
 
 }  // namespace cave

@@ -61,9 +61,11 @@ __device__ __forceinline__ void il_extent(uint32_t m, int &lo, int &hi)
 
 template <uint32_t FLAGS, int RPL>
 __global__ void __launch_bounds__(256) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
-                                                     StreamArgs a, StatPart *__restrict__ partials)
+                                                     StreamArgs a, StatPart *__restrict__ partials,
+                                                     const uint32_t *__restrict__ done)
 {
     __shared__ int32_t red[10 * 4];
+    if (done && *done) return;  // the manager already ended this simulation
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t lpr = 1u << a.lpr_log2;
@@ -286,18 +288,18 @@ StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t wa
 
 template <uint32_t FLAGS>
 static void launch_flags(const uint4 *in, uint4 *out, const StreamArgs &a, const StreamPlan &p, StatPart *parts,
-                         hipStream_t st)
+                         const uint32_t *done, hipStream_t st)
 {
     const dim3 grid(p.blocks), block(64 * p.waves);
     switch (p.rpl) {
-    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1>), grid, block, 0, st, in, out, a, parts); break;
-    case 2: hipLaunchKernelGGL((k_step_stream<FLAGS, 2>), grid, block, 0, st, in, out, a, parts); break;
-    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4>), grid, block, 0, st, in, out, a, parts); break;
+    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 2: hipLaunchKernelGGL((k_step_stream<FLAGS, 2>), grid, block, 0, st, in, out, a, parts, done); break;
+    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4>), grid, block, 0, st, in, out, a, parts, done); break;
     }
 }
 
 int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, const StreamPlan &p,
-                  StatPart *partials, hipStream_t st)
+                  StatPart *partials, hipStream_t st, const uint32_t *done)
 {
     if (p.rpl != 1 && p.rpl != 2 && p.rpl != 4) return fail(CAVE_E_INVALID, "rows per lane must be 1, 2 or 4");
     if (p.waves < 1 || p.waves > 4) return fail(CAVE_E_INVALID, "waves per block must be 1..4");
@@ -319,14 +321,14 @@ int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
     const uint4 *i4 = reinterpret_cast<const uint4 *>(in);
     uint4 *o4 = reinterpret_cast<uint4 *>(out);
     switch (rule.flags & 7u) {
-    case 1: launch_flags<1>(i4, o4, a, p, partials, st); break;
-    case 2: launch_flags<2>(i4, o4, a, p, partials, st); break;
-    case 3: launch_flags<3>(i4, o4, a, p, partials, st); break;
-    case 4: launch_flags<4>(i4, o4, a, p, partials, st); break;
-    case 5: launch_flags<5>(i4, o4, a, p, partials, st); break;
-    case 6: launch_flags<6>(i4, o4, a, p, partials, st); break;
-    case 7: launch_flags<7>(i4, o4, a, p, partials, st); break;
-    default: launch_flags<0>(i4, o4, a, p, partials, st); break;  // no neighbourhood: counts stay 0
+    case 1: launch_flags<1>(i4, o4, a, p, partials, done, st); break;
+    case 2: launch_flags<2>(i4, o4, a, p, partials, done, st); break;
+    case 3: launch_flags<3>(i4, o4, a, p, partials, done, st); break;
+    case 4: launch_flags<4>(i4, o4, a, p, partials, done, st); break;
+    case 5: launch_flags<5>(i4, o4, a, p, partials, done, st); break;
+    case 6: launch_flags<6>(i4, o4, a, p, partials, done, st); break;
+    case 7: launch_flags<7>(i4, o4, a, p, partials, done, st); break;
+    default: launch_flags<0>(i4, o4, a, p, partials, done, st); break;  // no neighbourhood: counts stay 0
     }
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;

@@ -1,0 +1,122 @@
+"""Batched evaluation (cave_batch_run) vs the CPU oracle's per-rule runs.
+
+Every simulation is compared field by field: the manager's GridInfo, the
+classification, the per-tick counters / bbox and the FNV-1a hash of the final
+cells.  Both engines (persistent and plain per-config) are checked, and both
+storage tiers of the persistent kernel (LDS-resident and global arena)."""
+import numpy as np
+import pytest
+
+import cave_amd as cv
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_run(sim: cv.Sim):
+    X, Y, Z = sim.dims
+    init = O.init_cells(X, Y, Z, sim.M, sim.spawn, sim.spawn_mode, sim.density, sim.seed)
+    if sim.sim_flags & cv.SIM_NO_EARLY_EXIT:
+        # step for exactly `ticks` ticks, manager semantics per tick
+        cells, rows = init, []
+        w0 = ((sim.M - 1) & 0xF) << 60 | sim.birth
+        w1 = sim.flags << 60 | sim.survival
+        for _ in range(sim.ticks):
+            cells, st = O.step(cells, X, Y, Z, w0, w1)
+            rows.append(st)
+        return None, rows, cells
+    t, info, rows, final = O.run_rule(X, Y, Z, sim.M, sim.flags, sim.birth, sim.survival, sim.ticks, init)
+    return info, rows, final
+
+
+def random_sims(n, seed, dims_choices, ticks=12, flags_choices=(1, 2, 3, 4, 5, 6, 7), p=0.2):
+    rng = np.random.default_rng(seed)
+    sims = []
+    for i in range(n):
+        dims = tuple(int(v) for v in dims_choices[i % len(dims_choices)])
+        flags = int(flags_choices[i % len(flags_choices)])
+        M = int(rng.integers(1, 16))
+        N = O.lib().orc_neighbor_count(flags)
+        birth = sum(1 << b for b in range(N) if rng.random() < p) or 1
+        surv = sum(1 << b for b in range(N) if rng.random() < p) or 2
+        spawn = tuple(max(1, d // 2) for d in dims)
+        sims.append(cv.Sim(dims=dims, M=M, flags=flags, ticks=ticks, birth=birth, survival=surv,
+                           spawn=spawn, spawn_mode=cv.SPAWN_RANDOM, density=0.3,
+                           seed=int(rng.integers(0, 2**31)), sim_flags=cv.SIM_FINAL_HASH))
+    return sims
+
+
+def check(sims, engine, per_tick=True):
+    print(f"batch of {len(sims)} engine={engine}", flush=True)
+    res, rows, _ = cv.batch_run(sims, engine=engine, per_tick=per_tick)
+    for i, (s, r) in enumerate(zip(sims, res)):
+        info, orows, final = oracle_run(s)
+        X, Y, Z = s.dims
+        if info is not None:
+            got = (r.info.totalNumberOfCellsThatDied, r.info.totalNumberOfCellsThatWereBorn,
+                   r.info.numberOfSimulationTicksSurvived, r.info.status, r.info.maxAliveCount,
+                   r.info.finalAliveCount)
+            want = (info.totalNumberOfCellsThatDied, info.totalNumberOfCellsThatWereBorn,
+                    info.numberOfSimulationTicksSurvived, info.status, info.maxAliveCount,
+                    info.finalAliveCount)
+            assert got == want, f"sim {i} {s}"
+            assert r.classification == O.lib().orc_classify(info, s.M), f"sim {i}"
+        assert r.ticks_run == len(orows), f"sim {i}"
+        if per_tick:
+            assert [t.as_tuple() for t in rows[i]] == [t.as_tuple() for t in orows], f"sim {i}"
+        assert r.last.as_tuple() == orows[-1].as_tuple(), f"sim {i}"
+        assert r.final_hash == O.canonical_hash(final, X, Y, Z), f"sim {i}"
+    return res
+
+
+@pytest.mark.parametrize("engine", [cv.ENGINE_PERSISTENT, cv.ENGINE_PLAIN])
+def test_small_grids_lds_tier(engine):
+    dims = [(1, 1, 1), (7, 5, 3), (33, 9, 4), (49, 12, 10), (16, 16, 16), (40, 3, 30), (64, 20, 2)]
+    check(random_sims(21, 1, dims), engine)
+
+
+@pytest.mark.parametrize("engine", [cv.ENGINE_PERSISTENT, cv.ENGINE_PLAIN])
+def test_global_tier_grids(engine):
+    # storage above the LDS budget: 6 planes x ceil(X/32) x Y x Z words > 159 KB
+    dims = [(70, 40, 30), (100, 33, 21), (65, 64, 64)]
+    check(random_sims(6, 2, dims, ticks=6), engine)
+
+
+def test_engines_agree_on_cfg3_slice():
+    # 49^3 FE, C in 2..5, <= 2 birth / survival bits from counts 1..18, 12^3 Bernoulli(0.3)
+    vb = cv.enumerate_valid_bitmasks(18, 2)
+    rng = cv.Rng(11)
+    sims = []
+    for c in range(6):
+        M = 1 + rng.next() % 4
+        b, s = vb[rng.next() % len(vb)], vb[rng.next() % len(vb)]
+        for seed in range(3):
+            sims.append(cv.Sim(dims=(49, 49, 49), M=M, flags=3, ticks=20, birth=b, survival=s,
+                               spawn=(12, 12, 12), spawn_mode=cv.SPAWN_RANDOM, density=0.3, seed=seed,
+                               sim_flags=cv.SIM_FINAL_HASH))
+    a = check(sims, cv.ENGINE_PERSISTENT)
+    b, _, _ = cv.batch_run(sims, engine=cv.ENGINE_PLAIN)
+    assert [x.final_hash for x in a] == [x.final_hash for x in b]
+
+
+def test_no_early_exit_runs_every_tick():
+    sims = random_sims(6, 3, [(20, 20, 20), (33, 17, 9)], ticks=15)
+    for s in sims:
+        s.sim_flags |= cv.SIM_NO_EARLY_EXIT
+    res, rows, _ = cv.batch_run(sims, per_tick=True)
+    for s, r, rr in zip(sims, res, rows):
+        _, orows, final = oracle_run(s)
+        assert r.ticks_run == s.ticks
+        assert [t.as_tuple() for t in rr] == [t.as_tuple() for t in orows]
+        assert r.final_hash == O.canonical_hash(final, *s.dims)
+
+
+def test_many_jobs_work_queue():
+    # more jobs than resident blocks: every result slot is filled exactly once
+    sims = random_sims(3000, 4, [(24, 24, 24), (12, 30, 17), (40, 40, 8)], ticks=5)
+    res, _, _ = cv.batch_run(sims, per_tick=False)
+    pick = list(range(0, 3000, 97))
+    for i in pick:
+        info, orows, final = oracle_run(sims[i])
+        assert res[i].info.numberOfSimulationTicksSurvived == info.numberOfSimulationTicksSurvived
+        assert res[i].final_hash == O.canonical_hash(final, *sims[i].dims)
