@@ -97,6 +97,65 @@ def cpu_step_baseline(target_s: float = 12.0):
             "sample": f"{ticks} tick(s) of the 512^3 FEV C=8 workload, scalar oracle, 1 thread"}
 
 
+def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
+    """Rule-search rate at 49^3 x 20 ticks (FE, C in 2..5, <= 2 + 2 rule bits,
+    32 seeds per config).  Persistent batched engine over the whole list; the
+    plain one-launch-per-config engine and the CPU oracle on bounded samples."""
+    import numpy as np
+
+    import cave_amd as cv
+    from cave_amd import workloads as W
+
+    specs = W.search_uniform(n_configs, seeds)
+    cells = 49 ** 3
+    cv.batch_run_np(specs[: seeds * 8])  # warm-up (code objects, allocator)
+    t0 = time.perf_counter()
+    res, dev_ms = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT)
+    wall = time.perf_counter() - t0
+    agg = W.aggregate_by_config(specs, res, seeds)
+    updates = int(res["ticks_run"].astype(np.int64).sum()) * cells
+    out = {
+        "metric": "rule-configs/sec at 49^3x20 ticks",
+        "value": n_configs / (dev_ms / 1e3),
+        "value_wall": n_configs / wall,
+        "unit": "configs/s",
+        "configs": n_configs, "seeds_per_config": seeds, "sims": len(specs),
+        "device_ms": dev_ms, "wall_ms": wall * 1e3,
+        "cell_updates": updates, "cell_updates_per_s": updates / (dev_ms / 1e3),
+        "early_exit": "reference status-1 early death on",
+        "classes": {"viable": int(agg["viable"].sum()), "possible": int(agg["possible"].sum()),
+                    "unviable": int(agg["unviable"].sum())},
+    }
+    sub = specs[: plain_configs * seeds]
+    pres, pms = cv.batch_run_np(sub, engine=cv.ENGINE_PLAIN)
+    assert np.array_equal(pres["ticks_run"], res["ticks_run"][: len(sub)]), "plain / persistent disagree"
+    out["plain"] = {"value": plain_configs / (pms / 1e3), "unit": "configs/s", "device_ms": pms,
+                    "sample": f"first {plain_configs} configs x {seeds} seeds, one step + one manager "
+                              "launch per sim per tick"}
+    out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        n_cpu, t0, cpu_updates, done = 0, time.perf_counter(), 0, 0
+        inits = {}
+        while time.perf_counter() - t0 < 10.0 and done < len(specs):
+            sp = specs[done]
+            M = int(sp["M"])
+            key = (M, int(sp["seed"]))
+            if key not in inits:
+                inits[key] = O.init_cells(49, 49, 49, M, (12, 12, 12), 0, 0.3, int(sp["seed"]))
+            t, info, _, _ = O.run_rule(49, 49, 49, M, 3, int(sp["birth"]), int(sp["survival"]), 20, inits[key])
+            cpu_updates += t * cells
+            done += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / seeds / dt, "unit": "configs/s",
+                               "cell_updates_per_s": cpu_updates / dt, "cores": 1, "kind": "port",
+                               "sample": f"first {done} sims ({done / seeds:.2f} configs) of the same list, "
+                                         "scalar oracle, 1 thread"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +163,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--search-configs", type=int, default=4096)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the whole grid lives on one MI355X and "
@@ -145,6 +205,8 @@ def main():
         cb = cpu_step_baseline()
         cb["host"] = f"{model} ({ncpu} logical CPUs visible)"
         out["cpu_baseline"] = cb
+    if not a.no_search:
+        out["search"] = run_search_bench(a.search_configs, cpu=not a.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -21,7 +21,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cave.h")
 CAVE_OK, CAVE_E_INVALID, CAVE_E_UNSUPPORTED, CAVE_E_NOMEM, CAVE_E_DEVICE, CAVE_E_BUDGET, CAVE_E_STATE = range(7)
 FACES, EDGES, CORNERS, WRAP = 1, 2, 4, 8
 SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER = 0, 1, 2
-KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM = 0, 1, 2
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE = 0, 1, 2, 3
 ENGINE_AUTO, ENGINE_PERSISTENT, ENGINE_PLAIN = 0, 1, 2
 SIM_NO_EARLY_EXIT, SIM_FINAL_HASH = 1, 2
 
@@ -335,6 +335,33 @@ class Sim:
         s.density = float(self.density)
         s.seed = self.seed
         s.flags = self.sim_flags
+
+
+# numpy mirrors of CaveSimSpec / CaveSimResult for large batches
+SIM_DTYPE = np.dtype({
+    "names": ["dims", "M", "flags", "ticks", "birth", "survival", "spawn", "spawn_mode", "density",
+              "seed", "sim_flags"],
+    "formats": [("<u4", 3), "<u4", "<u4", "<u4", "<u8", "<u8", ("<u4", 3), "<u4", "<f8", "<u4", "<u4"],
+    "offsets": [0, 12, 16, 20, 24, 32, 40, 52, 56, 64, 68],
+    "itemsize": 72})
+RESULT_DTYPE = np.dtype({
+    "names": ["died", "born", "ticks_survived", "status", "max_alive", "final_alive", "classification",
+              "ticks_run", "last", "final_hash"],
+    "formats": ["<u4", "<u4", "<u4", "<u4", "<u4", "<u4", "<u4", "<u4", ("<u4", 12), "<u8"],
+    "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 80],
+    "itemsize": 88})
+
+
+def batch_run_np(specs: np.ndarray, engine=ENGINE_AUTO, block_threads=0):
+    """Evaluates a SIM_DTYPE array; returns (RESULT_DTYPE array, device ms)."""
+    specs = np.ascontiguousarray(specs, dtype=SIM_DTYPE)
+    res = np.zeros(len(specs), dtype=RESULT_DTYPE)
+    opt = BatchOptions()
+    opt.engine = engine
+    opt.block_threads = block_threads
+    check(lib().cave_batch_run(specs.ctypes.data_as(C.POINTER(SimSpec)), len(specs),
+                               res.ctypes.data_as(C.POINTER(SimResult)), C.byref(opt)))
+    return res, opt.device_ms
 
 
 def batch_run(sims, engine=ENGINE_AUTO, per_tick=False, block_threads=0):

@@ -238,13 +238,14 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
     std::vector<int> use_stream(n);
     std::vector<StreamPlan> plans(n);
     for (size_t i = 0; i < n; i++) {
-        GridGeom g{specs[i].dims[0], specs[i].dims[1], specs[i].dims[2], 4u * ((specs[i].dims[0] + 31u) / 32u)};
+        GridGeom g{specs[i].dims[0], specs[i].dims[1], specs[i].dims[2], 4u * ((specs[i].dims[0] + 31u) / 32u), 1u};
         use_stream[i] = stream_supported(g);
         if (use_stream[i]) {
             plans[i] = stream_plan(g, 0, 0, 0);
             max_parts = std::max(max_parts, plans[i].blocks);
         } else {
             g.pitch = ref_words_per_row(g.X);
+            g.planar = 0;
             max_parts = std::max(max_parts, gather_blocks(g));
         }
         geo[i] = g;
@@ -290,8 +291,15 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
             CAVE_HIP(hipStreamSynchronize(st));
             const uint32_t upr = ref_words_per_row(g.X);
             std::vector<uint32_t> cells((size_t)upr * g.Y * g.Z);
-            CAVE_HIP(hipMemcpy2DAsync(cells.data(), (size_t)upr * 4, bufs[hs[i].info.numberOfSimulationTicksSurvived & 1u],
-                                      (size_t)g.pitch * 4, (size_t)upr * 4, (size_t)g.Y * g.Z, hipMemcpyDeviceToHost, st));
+            uint32_t *fin = bufs[hs[i].info.numberOfSimulationTicksSurvived & 1u];
+            uint32_t *idle = bufs[(hs[i].info.numberOfSimulationTicksSurvived + 1u) & 1u];
+            if (g.planar) {
+                if ((rc = launch_planar_to_ref(fin, idle, g, st))) return rc;
+                CAVE_HIP(hipMemcpyAsync(cells.data(), idle, cells.size() * 4, hipMemcpyDeviceToHost, st));
+            } else {
+                CAVE_HIP(hipMemcpy2DAsync(cells.data(), (size_t)upr * 4, fin, (size_t)g.pitch * 4, (size_t)upr * 4,
+                                          (size_t)g.Y * g.Z, hipMemcpyDeviceToHost, st));
+            }
             CAVE_HIP(hipStreamSynchronize(st));
             hashes[i] = fnv_ref_layout(cells.data(), g.X, g.Y, g.Z);
         }

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -82,7 +84,7 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     if (spec->spawn_mode > CAVE_SPAWN_BUFFER) return fail(CAVE_E_INVALID, "bad spawn_mode");
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER && !spec->initial_cells)
         return fail(CAVE_E_INVALID, "CAVE_SPAWN_BUFFER needs initial_cells");
-    if (spec->kernel > CAVE_KERNEL_STREAM) return fail(CAVE_E_INVALID, "bad kernel selector");
+    if (spec->kernel > CAVE_KERNEL_STREAM_NIBBLE) return fail(CAVE_E_INVALID, "bad kernel selector");
 
     CaveStep *h = new (std::nothrow) CaveStep();
     if (!h) return fail(CAVE_E_NOMEM, "host allocation failed");
@@ -96,14 +98,20 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     GridGeom sg = h->g;
     sg.pitch = 4u * ((h->g.X + 31u) / 32u);
     const bool stream_ok = stream_supported(sg);
-    if (spec->kernel == CAVE_KERNEL_STREAM && !stream_ok) {
+    const bool want_stream = spec->kernel == CAVE_KERNEL_STREAM || spec->kernel == CAVE_KERNEL_STREAM_NIBBLE;
+    if (want_stream && !stream_ok) {
         release(h);
         return fail(CAVE_E_UNSUPPORTED, "the streaming kernel needs X <= 512");
     }
-    h->kernel = (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) ? CAVE_KERNEL_GATHER : CAVE_KERNEL_STREAM;
-    if (h->kernel == CAVE_KERNEL_STREAM) {
+    if (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) h->kernel = CAVE_KERNEL_GATHER;
+    else h->kernel = spec->kernel == CAVE_KERNEL_STREAM_NIBBLE ? CAVE_KERNEL_STREAM_NIBBLE : CAVE_KERNEL_STREAM;
+    if (h->kernel != CAVE_KERNEL_GATHER) {
+        sg.planar = h->kernel == CAVE_KERNEL_STREAM ? 1u : 0u;
         h->g = sg;
-        h->plan = stream_plan(h->g, 0, 0, 0);
+        // CAVE_STREAM_PLAN="rows_per_lane,z_planes,waves" overrides the default tiling
+        uint32_t rpl = 0, tz = 0, waves = 0;
+        if (const char *env = std::getenv("CAVE_STREAM_PLAN")) std::sscanf(env, "%u,%u,%u", &rpl, &tz, &waves);
+        h->plan = stream_plan(h->g, rpl, tz, waves);
         h->nparts = h->plan.blocks;
     } else {
         h->g.pitch = ref_words_per_row(h->g.X);
@@ -130,10 +138,9 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     CK(hipMalloc(&h->partials, sizeof(StatPart) * (size_t)h->nparts * h->chunk));
     CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
-        const uint32_t upr = ref_words_per_row(h->g.X);
-        CK(hipMemsetAsync(h->cur, 0, bytes, h->stream));
-        CK(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, spec->initial_cells, (size_t)upr * 4, (size_t)upr * 4,
-                            (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
+        CK(hipStreamSynchronize(h->stream));
+        rc = cave_step_upload(h, spec->initial_cells, cave_cell_words(h->g.X, h->g.Y, h->g.Z));
+        if (rc) { release(h); return rc; }
     } else {
         const uint32_t dims[3] = {h->g.X, h->g.Y, h->g.Z};
         const DevInit init = make_init(dims, spec->spawn, spec->spawn_mode, spec->density, spec->seed);
@@ -156,7 +163,7 @@ int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *
         CAVE_HIP(hipEventRecord(h->ev0, h->stream));
         for (uint32_t t = 0; t < n; t++) {
             StatPart *parts = h->partials + (size_t)t * h->nparts;
-            int rc = h->kernel == CAVE_KERNEL_STREAM
+            int rc = h->kernel != CAVE_KERNEL_GATHER
                          ? launch_stream(h->cur, h->nxt, h->g, h->rule, h->plan, parts, h->stream)
                          : launch_gather(h->cur, h->nxt, h->g, h->rule, parts, h->stream);
             if (rc) return rc;
@@ -183,8 +190,15 @@ int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words)
     if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
     const uint32_t upr = ref_words_per_row(h->g.X);
     if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
-    CAVE_HIP(hipMemcpy2DAsync(cells, (size_t)upr * 4, h->cur, (size_t)h->g.pitch * 4, (size_t)upr * 4,
-                              (size_t)h->g.Y * h->g.Z, hipMemcpyDeviceToHost, h->stream));
+    if (h->g.planar) {
+        // convert into the idle ping-pong buffer, then copy the reference layout out
+        int rc = launch_planar_to_ref(h->cur, h->nxt, h->g, h->stream);
+        if (rc) return rc;
+        CAVE_HIP(hipMemcpyAsync(cells, h->nxt, (size_t)upr * h->g.Y * h->g.Z * 4, hipMemcpyDeviceToHost, h->stream));
+    } else {
+        CAVE_HIP(hipMemcpy2DAsync(cells, (size_t)upr * 4, h->cur, (size_t)h->g.pitch * 4, (size_t)upr * 4,
+                                  (size_t)h->g.Y * h->g.Z, hipMemcpyDeviceToHost, h->stream));
+    }
     CAVE_HIP(hipStreamSynchronize(h->stream));
     return CAVE_OK;
 }
@@ -194,9 +208,17 @@ int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words)
     if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
     const uint32_t upr = ref_words_per_row(h->g.X);
     if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
-    CAVE_HIP(hipMemsetAsync(h->cur, 0, dev_words(h->g) * 4, h->stream));
-    CAVE_HIP(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, cells, (size_t)upr * 4, (size_t)upr * 4,
-                              (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
+    if (h->g.planar) {
+        CAVE_HIP(hipMemcpyAsync(h->nxt, cells, (size_t)upr * h->g.Y * h->g.Z * 4, hipMemcpyHostToDevice, h->stream));
+        int rc = launch_ref_to_planar(h->nxt, h->cur, h->g, h->stream);
+        if (rc) return rc;
+    } else {
+        CAVE_HIP(hipMemsetAsync(h->cur, 0, dev_words(h->g) * 4, h->stream));
+        CAVE_HIP(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, cells, (size_t)upr * 4, (size_t)upr * 4,
+                                  (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
+        int rc = launch_sanitize_nibbles(h->cur, h->g, h->stream);
+        if (rc) return rc;
+    }
     CAVE_HIP(hipStreamSynchronize(h->stream));
     return CAVE_OK;
 }

@@ -174,6 +174,33 @@ CAVE_HD void apply_rule(const Bits<5> &n, uint32_t born_lut, uint32_t surv_lut, 
     }
 }
 
+// Same tables for R rows at once: the scalar loop over the set counts (a
+// wavefront-uniform value) is shared by the rows.
+template <int R>
+CAVE_HD void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_lut, uint32_t *tb, uint32_t *ts)
+{
+#pragma unroll
+    for (int r = 0; r < R; r++) tb[r] = ts[r] = 0u;
+    uint32_t u = born_lut | surv_lut;
+    while (u) {
+        const uint32_t v = (uint32_t)__builtin_ctz(u);
+        u &= u - 1u;
+        const uint32_t bm = 0u - ((born_lut >> v) & 1u);
+        const uint32_t sm = 0u - ((surv_lut >> v) & 1u);
+        uint32_t inv[5];
+#pragma unroll
+        for (int b = 0; b < 5; b++) inv[b] = ((v >> b) & 1u) ? 0u : ~0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            uint32_t mt = ~0u;
+#pragma unroll
+            for (int b = 0; b < 5; b++) mt &= n[r].b[b] ^ inv[b];
+            tb[r] |= mt & bm;
+            ts[r] |= mt & sm;
+        }
+    }
+}
+
 // ---- state update ----
 // new = M where set_m, s-1 where s != 0 and !set_m, 0 elsewhere
 // (searchSimulationCube.slang:178-211).  mk[k] = ~0 where bit k of M is set.

@@ -222,9 +222,33 @@ int launch_plain_manager(const StatPart *partials, uint32_t nparts, PlainState *
 
 int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st)
 {
+    if (g.planar) return launch_init_planar(cells, g, init, M, st);
     const size_t n = (size_t)g.pitch * g.Y * g.Z;
     const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 65535u * 4u);
     hipLaunchKernelGGL(k_init_cells, dim3(blocks), dim3(256), 0, st, cells, g, init, M & 15u);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+// Zeroes every nibble at x >= X (row padding) of a row-pitched nibble layout:
+// the reference writes padding as 0 (searchSimulationCube.slang:161-166) and
+// never reads it as a neighbour, so caller-supplied cells are normalised once.
+__global__ void k_sanitize_nibbles(uint32_t *cells, GridGeom g)
+{
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < n; w += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x0 = (uint32_t)(w % g.pitch) * 8u;
+        if (x0 + 8u <= g.X) continue;
+        const uint32_t keep = x0 >= g.X ? 0u : (0xFFFFFFFFu >> (4u * (8u - (g.X - x0))));
+        cells[w] &= keep;
+    }
+}
+
+int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st)
+{
+    const size_t n = (size_t)g.pitch * g.Y * g.Z;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 65535u * 4u);
+    hipLaunchKernelGGL(k_sanitize_nibbles, dim3(blocks), dim3(256), 0, st, cells, g);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

@@ -17,10 +17,18 @@ namespace cave {
 struct GridGeom {
     uint32_t X, Y, Z;
     uint32_t pitch;
+    uint32_t planar = 0;  // 1: bit-planar 32-cell groups (pitch = 4 words per group)
 };
 
+// Builds the initial cells in the geometry's layout (nibble or planar).
 int launch_init(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st);
+int launch_init_planar(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st);
+// Layout conversion between the reference nibble layout (ceil(X/8) words per
+// row) and the planar layout of g.
+int launch_ref_to_planar(const uint32_t *ref, uint32_t *planar, const GridGeom &g, hipStream_t st);
+int launch_planar_to_ref(const uint32_t *planar, uint32_t *ref, const GridGeom &g, hipStream_t st);
 
+int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st);
 uint32_t gather_blocks(const GridGeom &g);
 // done (nullable): device flag; when set the launch does nothing (the plain
 // per-config driver ends a simulation on the device without a host round trip)

@@ -1,8 +1,17 @@
 // step_stream.hip -- bit-sliced z-streaming step kernel for one large grid.
 //
+// Cell layouts (4 bits per cell either way):
+//   * planar (default): a row is split into 32-cell groups; a group is one
+//     128-bit word holding the four bit planes of its 32 cells (word k, bit c
+//     = bit k of the state of cell 32g+c).  The reference's nibble layout is
+//     restored at the API boundary (upload / download / init).
+//   * nibble: the reference layout itself (8 nibbles per 32-bit word), rows
+//     padded to whole 32-cell groups; converted to planes in registers after
+//     the load and back before the store.
+//
 // Work decomposition (one tick per launch):
-//   * a lane owns a 32-cell x-group (four packed words, one 128-bit load) of
-//     RPL consecutive y-rows, and walks a z-range of TZ planes;
+//   * a lane owns a 32-cell x-group (one 128-bit load / store) of RPL
+//     consecutive y-rows, and walks a z-range of TZ planes;
 //   * LPR lanes cover one x-row, so an x-row of up to 512 cells sits in one
 //     16-lane DPP row: the x-neighbour live bits cross lanes with row_shr /
 //     row_shl DPP moves;
@@ -10,13 +19,14 @@
 //     just above / below a slot come from the neighbouring slot with a lane
 //     shuffle, and only the wave's two edge slots load a halo row;
 //   * along z the kernel keeps the per-row partial sums S0 / S1 of the last
-//     planes in registers, so each packed word is loaded once per tick (plus
-//     the y / z halo of the wave tile) and stored once.
-// Cells are converted to four bit planes right after the load; neighbour
-// counts, the rule and the state update are bit-sliced (32 cells per 32-bit
-// instruction), and the result is converted back to packed nibbles for the
-// store.  Statistics are reduced per lane, then per wave with cross-lane ops,
-// then per block; one partial per block per tick, no atomics.
+//     planes in registers, so each group is loaded once per tick (plus the
+//     y / z halo of the wave tile) and stored once.  The z loop is unrolled by
+//     two with two alternating register sets, so nothing is copied between
+//     iterations.
+// Neighbour counts, the rule and the state update are bit-sliced (32 cells
+// per 32-bit instruction).  Statistics are reduced per lane, then per wave
+// with cross-lane ops, then per block: one partial per block per tick, no
+// atomics.
 #include "bitslice.h"
 #include "step_kernels.h"
 
@@ -24,13 +34,13 @@ namespace cave {
 
 struct StreamArgs {
     uint32_t X, Y, Z;
-    uint32_t G;          // 32-cell groups per row (pitch / 4)
+    uint32_t G;          // 32-cell groups per row
     uint32_t lpr_log2;   // lanes per x-row = 1 << lpr_log2
     uint32_t rows_per_wave;
     uint32_t tz;
     uint32_t ytiles;
     uint32_t total_waves;
-    uint32_t last_mask;  // valid cells of the last group (interleaved order)
+    uint32_t last_mask;  // valid cells of the last group of a row (in the layout's bit order)
     RuleConsts rc;
 };
 
@@ -44,10 +54,16 @@ __device__ __forceinline__ uint32_t dpp_from_next_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, true);
 }
 
-// Lowest / highest cell index (0..31) set in an interleaved-order mask.
-__device__ __forceinline__ void il_extent(uint32_t m, int &lo, int &hi)
+// Lowest / highest cell index (0..31) set in a mask.
+template <bool PLANAR>
+__device__ __forceinline__ void mask_extent(uint32_t m, int &lo, int &hi)
 {
-    lo = 64;
+    if (PLANAR) {
+        lo = (int)__builtin_ctz(m);
+        hi = 31 - (int)__builtin_clz(m);
+        return;
+    }
+    lo = 64;  // interleaved order: bit 4i+j = cell 8j+i
     hi = -1;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -59,194 +75,338 @@ __device__ __forceinline__ void il_extent(uint32_t m, int &lo, int &hi)
     }
 }
 
-template <uint32_t FLAGS, int RPL>
+template <int RPL>
+struct Carry {
+    uint4 w[RPL + 1];      // raw groups of the plane the step receiving this set consumes
+    uint32_t P[RPL][4];    // state planes of the last input plane
+    uint32_t A[RPL];       // its live masks
+    uint32_t NZ[RPL];      // its non-zero masks
+    Bits<4> S1p[RPL];      // S1 of the last input plane
+    Bits<5> Q[RPL];        // S1 of the plane before it + S0 of the last input plane
+};
+
+template <uint32_t FLAGS, int RPL, bool PLANAR>
+struct StreamLane {
+    // Buffer resources over the two cell buffers: a 128-bit access whose byte
+    // offset is at or beyond the buffer size reads zeros / is dropped by the
+    // hardware range check, which is how rows and planes outside the grid
+    // (y or z halo beyond the faces, groups past the row end) are handled
+    // without branches.
+    __amdgpu_buffer_rsrc_t rin, rout;
+    const StreamArgs &a;
+    uint32_t lane, xg, slot, nslots, lpr, vmask;
+    int y_first, z0, z1;
+    bool yok[RPL + 1];
+    uint32_t roff[RPL + 1];  // byte offset of the row within a z plane (kOutOfRange if none)
+    uint32_t plane_bytes;
+    // statistics
+    uint32_t surv = 0, born = 0, dying = 0, nrows = 0, colmask = 0;
+    uint32_t yany[RPL];
+    int zmin = 0x7fffffff, zmax = -1;
+
+    __device__ __forceinline__ uint4 load(int r, int z) const
+    {   // z = -1 wraps to a huge offset and z = Z lands at the buffer end: both read zeros
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rin, (int)(roff[r] + (uint32_t)z * plane_bytes), 0, 0);
+        return make_uint4(v[0], v[1], v[2], v[3]);
+    }
+
+    __device__ __forceinline__ void store(int r, int z, const uint32_t P[4]) const
+    {
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        const v4u v = {P[0], P[1], P[2], P[3]};
+        __builtin_amdgcn_raw_buffer_store_b128(v, rout, (int)(roff[r] + (uint32_t)z * plane_bytes), 0, 0);
+    }
+
+    __device__ __forceinline__ void planes_of(const uint4 &w, uint32_t P[4]) const
+    {
+        P[0] = w.x;
+        P[1] = w.y;
+        P[2] = w.z;
+        P[3] = w.w;
+        if (!PLANAR) transpose4x4(P[0], P[1], P[2], P[3]);
+    }
+
+    // One z step: consume plane zi (ci.w), refill ci.w with plane zi+2 (the
+    // step after next runs on the same register set: two planes in flight),
+    // finalize plane zi-1.
+    __device__ __forceinline__ void step(int zi, Carry<RPL> &ci, Carry<RPL> &co)
+    {
+        const RuleConsts &rc = a.rc;
+        RowSums rs[RPL + 1];
+        uint4 wcur[RPL + 1];
+#pragma unroll
+        for (int r = 0; r <= RPL; r++) wcur[r] = ci.w[r];
+        if (zi < z1) {
+#pragma unroll
+            for (int r = 0; r <= RPL; r++) co.w[r] = load(r, zi + 1);
+        }
+#pragma unroll
+        for (int r = 0; r <= RPL; r++) {
+            uint32_t P[4];
+            planes_of(wcur[r], P);
+            const uint32_t A = live_mask(P, rc.nm, rc.en);
+            if (r < RPL) {
+                co.P[r][0] = P[0];
+                co.P[r][1] = P[1];
+                co.P[r][2] = P[2];
+                co.P[r][3] = P[3];
+                co.A[r] = A;
+                co.NZ[r] = nonzero_mask(P);
+            }
+            uint32_t prev = dpp_from_prev_lane(A);
+            uint32_t next = dpp_from_next_lane(A);
+            prev = xg == 0u ? 0u : prev;
+            next = xg == lpr - 1u ? 0u : next;
+            const uint32_t Lm = PLANAR ? shift_left_neighbor_nat(A, prev) : shift_left_neighbor_il(A, prev);
+            const uint32_t Rm = PLANAR ? shift_right_neighbor_nat(A, next) : shift_right_neighbor_il(A, next);
+            rs[r] = row_sums(A, Lm, Rm);
+        }
+
+        // neighbour rows across slot boundaries
+        Bits<2> k1_up, k2_up, k1_dn, k2_dn;
+        {
+            const Bits<2> k1_last = kind1<FLAGS>(rs[RPL - 1]), k2_last = kind2<FLAGS>(rs[RPL - 1]);
+            const Bits<2> k1_first = kind1<FLAGS>(rs[0]), k2_first = kind2<FLAGS>(rs[0]);
+            const Bits<2> k1_h = kind1<FLAGS>(rs[RPL]), k2_h = kind2<FLAGS>(rs[RPL]);
+            const int src_up = ((int)lane - (int)lpr) & 63;  // slot - 1, same xg
+            const int src_dn = ((int)lane + (int)lpr) & 63;  // slot + 1, same xg
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t u1 = (uint32_t)__shfl((int)k1_last.b[k], src_up, 64);
+                const uint32_t u2 = (uint32_t)__shfl((int)k2_last.b[k], src_up, 64);
+                const uint32_t d1 = (uint32_t)__shfl((int)k1_first.b[k], src_dn, 64);
+                const uint32_t d2 = (uint32_t)__shfl((int)k2_first.b[k], src_dn, 64);
+                k1_up.b[k] = slot == 0 ? k1_h.b[k] : u1;
+                k2_up.b[k] = slot == 0 ? k2_h.b[k] : u2;
+                k1_dn.b[k] = slot == nslots - 1 ? k1_h.b[k] : d1;
+                k2_dn.b[k] = slot == nslots - 1 ? k2_h.b[k] : d2;
+            }
+        }
+
+        // plane partial sums of plane zi
+        Bits<4> S1c[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const Bits<2> u1 = r == 0 ? k1_up : kind1<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
+            const Bits<2> u2 = r == 0 ? k2_up : kind2<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
+            const Bits<2> d1 = r == RPL - 1 ? k1_dn : kind1<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
+            const Bits<2> d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
+            const Bits<4> S0c = add3<4>(u1, kind0<FLAGS>(rs[r]), d1);
+            S1c[r] = add3<4>(u2, kind1<FLAGS>(rs[r]), d2);
+            co.Q[r] = add3<5>(ci.S1p[r], S0c, Bits<1>{{0u}});
+            co.S1p[r] = S1c[r];
+        }
+
+        // finalize plane z = zi - 1 (uniform over the wave)
+        const int z = zi - 1;
+        if (z >= z0) {
+            Bits<5> n[RPL];
+            uint32_t tb[RPL], ts[RPL];
+#pragma unroll
+            for (int r = 0; r < RPL; r++) n[r] = add3<5>(ci.Q[r], S1c[r], Bits<1>{{0u}});
+            apply_rule_rows<RPL>(n, rc.born_lut, rc.surv_lut, tb, ts);
+            uint32_t any = 0u;
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const uint32_t valid = yok[r] ? vmask : 0u;
+                nrows += yok[r] ? 1u : 0u;
+                const uint32_t A = ci.A[r], NZ = ci.NZ[r];
+                const uint32_t set_m = valid & ((A & ts[r]) | (~NZ & tb[r]));
+                const uint32_t sv = set_m & A;
+                surv += __popc(sv);
+                born += __popc(set_m ^ sv);
+                dying += __popc(NZ & ~sv);
+                yany[r] |= set_m;
+                any |= set_m;
+                uint32_t P[4] = {ci.P[r][0], ci.P[r][1], ci.P[r][2], ci.P[r][3]};
+                update_planes(P, set_m, NZ, rc.mk);
+                if (!PLANAR) transpose4x4(P[0], P[1], P[2], P[3]);
+                store(r, z, P);
+            }
+            colmask |= any;
+            if (any) {
+                zmin = min(zmin, z);
+                zmax = max(zmax, z);
+            }
+        }
+    }
+};
+
+constexpr uint32_t kOutOfRange = 0x80000000u;  // any row offset this large is outside every buffer
+constexpr uint32_t kBufferDword3 = 0x00020000u;  // raw buffer resource word 3 on gfx9-family parts
+
+template <uint32_t FLAGS, int RPL, bool PLANAR>
 __global__ void __launch_bounds__(256) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
                                                      StreamArgs a, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
 {
     __shared__ int32_t red[10 * 4];
     if (done && *done) return;  // the manager already ended this simulation
-    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t lpr = 1u << a.lpr_log2;
-    const uint32_t xg = lane & (lpr - 1u);
-    const uint32_t slot = lane >> a.lpr_log2;
-    const uint32_t nslots = 64u >> a.lpr_log2;
     StatAcc acc;
 
     if (gwave < a.total_waves) {
+        const uint32_t buf_bytes = a.G * a.Y * a.Z * 16u;
+        StreamLane<FLAGS, RPL, PLANAR> L{__builtin_amdgcn_make_buffer_rsrc((void *)in, 0, (int)buf_bytes, kBufferDword3),
+                                         __builtin_amdgcn_make_buffer_rsrc((void *)out, 0, (int)buf_bytes, kBufferDword3),
+                                         a};
+        L.lane = threadIdx.x & 63u;
+        L.lpr = 1u << a.lpr_log2;
+        L.xg = L.lane & (L.lpr - 1u);
+        L.slot = L.lane >> a.lpr_log2;
+        L.nslots = 64u >> a.lpr_log2;
         const uint32_t ytile = gwave % a.ytiles;
         const uint32_t ztile = gwave / a.ytiles;
-        const int y_first = (int)(ytile * a.rows_per_wave + slot * RPL);
-        const int z0 = (int)(ztile * a.tz);
-        const int z1 = min(z0 + (int)a.tz, (int)a.Z);
-        const bool gvalid = xg < a.G;
-        const uint32_t vmask = !gvalid ? 0u : (xg + 1u < a.G ? ~0u : a.last_mask);
-        const int y_halo = slot == 0 ? y_first - 1 : (slot == nslots - 1 ? y_first + RPL : -1);
-        const RuleConsts rc = a.rc;
-        const size_t row_stride = a.G;           // uint4 per row
-        const size_t plane_stride = (size_t)a.G * a.Y;
+        L.y_first = (int)(ytile * a.rows_per_wave + L.slot * RPL);
+        L.z0 = (int)(ztile * a.tz);
+        L.z1 = min(L.z0 + (int)a.tz, (int)a.Z);
+        const bool gvalid = L.xg < a.G;
+        L.vmask = !gvalid ? 0u : (L.xg + 1u < a.G ? ~0u : a.last_mask);
+        const int y_halo = L.slot == 0 ? L.y_first - 1 : (L.slot == L.nslots - 1 ? L.y_first + RPL : -1);
+        L.plane_bytes = a.G * a.Y * 16u;
+#pragma unroll
+        for (int r = 0; r <= RPL; r++) {
+            const int y = r < RPL ? L.y_first + r : y_halo;
+            L.yok[r] = gvalid && y >= 0 && y < (int)a.Y;
+            L.roff[r] = L.yok[r] ? ((uint32_t)y * a.G + L.xg) * 16u : kOutOfRange;
+            if (r < RPL) L.yany[r] = 0u;
+        }
 
-        auto row_ok = [&](int y, int z) {
-            return gvalid && y >= 0 && y < (int)a.Y && z >= 0 && z < (int)a.Z;
-        };
-        auto load = [&](int y, int z) -> uint4 {
-            uint4 w = make_uint4(0u, 0u, 0u, 0u);
-            if (row_ok(y, z)) w = in[(size_t)z * plane_stride + (size_t)y * row_stride + xg];
-            return w;
-        };
-
-        // registers carried along z, per own row
-        uint32_t Pprev[RPL][4];
-        uint32_t Aprev[RPL], NZprev[RPL];
-        Bits<4> S1pp[RPL], S1p[RPL], S0p[RPL];
-        uint32_t yany[RPL];
-        uint32_t colmask = 0u;
-        int zmin = 0x7fffffff, zmax = -1;
+        Carry<RPL> c0, c1;
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
-            for (int k = 0; k < 4; k++) Pprev[r][k] = 0u;
-            Aprev[r] = NZprev[r] = 0u;
-            for (int k = 0; k < 4; k++) S1pp[r].b[k] = S1p[r].b[k] = S0p[r].b[k] = 0u;
-            yany[r] = 0u;
+            for (int k = 0; k < 4; k++) c0.P[r][k] = 0u;
+            c0.A[r] = c0.NZ[r] = 0u;
+            for (int k = 0; k < 4; k++) c0.S1p[r].b[k] = 0u;
+            for (int k = 0; k < 5; k++) c0.Q[r].b[k] = 0u;
+        }
+#pragma unroll
+        for (int r = 0; r <= RPL; r++) {
+            c0.w[r] = L.load(r, L.z0 - 1);
         }
 
-        // software pipeline: the words of plane zi are loaded one step ahead
-        uint4 wnext[RPL + 1];
-#pragma unroll
-        for (int r = 0; r < RPL; r++) wnext[r] = load(y_first + r, z0 - 1);
-        wnext[RPL] = load(y_halo, z0 - 1);
-
-        for (int zi = z0 - 1; zi <= z1; zi++) {
-            uint4 wcur[RPL + 1];
-#pragma unroll
-            for (int r = 0; r <= RPL; r++) wcur[r] = wnext[r];
-            if (zi < z1) {
-#pragma unroll
-                for (int r = 0; r < RPL; r++) wnext[r] = load(y_first + r, zi + 1);
-                wnext[RPL] = load(y_halo, zi + 1);
-            }
-
-            // planes, live masks and x-sums of the own rows and the halo row
-            uint32_t Pcur[RPL][4];
-            uint32_t Acur[RPL], NZcur[RPL];
-            RowSums rs[RPL + 1];
-#pragma unroll
-            for (int r = 0; r <= RPL; r++) {
-                uint32_t p0 = wcur[r].x, p1 = wcur[r].y, p2 = wcur[r].z, p3 = wcur[r].w;
-                transpose4x4(p0, p1, p2, p3);
-                const uint32_t P[4] = {p0, p1, p2, p3};
-                const uint32_t A = live_mask(P, rc.nm, rc.en);
-                if (r < RPL) {
-                    Pcur[r][0] = p0; Pcur[r][1] = p1; Pcur[r][2] = p2; Pcur[r][3] = p3;
-                    Acur[r] = A;
-                    NZcur[r] = nonzero_mask(P);
-                }
-                uint32_t prev = dpp_from_prev_lane(A);
-                uint32_t next = dpp_from_next_lane(A);
-                prev = xg == 0u ? 0u : prev;
-                next = xg == lpr - 1u ? 0u : next;
-                rs[r] = row_sums(A, shift_left_neighbor_il(A, prev), shift_right_neighbor_il(A, next));
-            }
-
-            // neighbour rows across slot boundaries
-            Bits<2> k1_up, k2_up, k1_dn, k2_dn;
-            {
-                const Bits<2> k1_last = kind1<FLAGS>(rs[RPL - 1]), k2_last = kind2<FLAGS>(rs[RPL - 1]);
-                const Bits<2> k1_first = kind1<FLAGS>(rs[0]), k2_first = kind2<FLAGS>(rs[0]);
-                const Bits<2> k1_h = kind1<FLAGS>(rs[RPL]), k2_h = kind2<FLAGS>(rs[RPL]);
-                const int src_up = (int)lane - (int)lpr;   // slot - 1, same xg
-                const int src_dn = (int)lane + (int)lpr;   // slot + 1, same xg
-#pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    const uint32_t u1 = (uint32_t)__shfl((int)k1_last.b[k], src_up & 63, 64);
-                    const uint32_t u2 = (uint32_t)__shfl((int)k2_last.b[k], src_up & 63, 64);
-                    const uint32_t d1 = (uint32_t)__shfl((int)k1_first.b[k], src_dn & 63, 64);
-                    const uint32_t d2 = (uint32_t)__shfl((int)k2_first.b[k], src_dn & 63, 64);
-                    k1_up.b[k] = slot == 0 ? k1_h.b[k] : u1;
-                    k2_up.b[k] = slot == 0 ? k2_h.b[k] : u2;
-                    k1_dn.b[k] = slot == nslots - 1 ? k1_h.b[k] : d1;
-                    k2_dn.b[k] = slot == nslots - 1 ? k2_h.b[k] : d2;
-                }
-            }
-
-            // plane partial sums of plane zi
-            Bits<4> S0c[RPL], S1c[RPL];
-#pragma unroll
-            for (int r = 0; r < RPL; r++) {
-                const Bits<2> u1 = r == 0 ? k1_up : kind1<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
-                const Bits<2> u2 = r == 0 ? k2_up : kind2<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
-                const Bits<2> d1 = r == RPL - 1 ? k1_dn : kind1<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
-                const Bits<2> d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
-                S0c[r] = add3<4>(u1, kind0<FLAGS>(rs[r]), d1);
-                S1c[r] = add3<4>(u2, kind1<FLAGS>(rs[r]), d2);
-            }
-
-            // finalize plane z = zi - 1
-            const int z = zi - 1;
-            if (z >= z0) {
-                uint32_t any = 0u;
-#pragma unroll
-                for (int r = 0; r < RPL; r++) {
-                    const int y = y_first + r;
-                    const Bits<5> n = add3<5>(S1pp[r], S0p[r], S1c[r]);
-                    uint32_t tb, ts;
-                    apply_rule(n, rc.born_lut, rc.surv_lut, tb, ts);
-                    const bool rv = row_ok(y, z);
-                    const uint32_t valid = rv ? vmask : 0u;
-                    const uint32_t A = Aprev[r], NZ = NZprev[r];
-                    const uint32_t zero = ~NZ;
-                    const uint32_t set_m = valid & ((A & ts) | (zero & tb));
-                    const uint32_t surv = set_m & A;
-                    const uint32_t born = set_m & ~A;
-                    const uint32_t dying = NZ & ~surv & valid;
-                    acc.surv += __popc(surv);
-                    acc.born += __popc(born);
-                    acc.dying += __popc(dying);
-                    acc.dead += __popc(valid) - __popc(dying) - __popc(surv) - __popc(born);
-                    colmask |= set_m;
-                    yany[r] |= set_m;
-                    any |= set_m;
-                    uint32_t P[4] = {Pprev[r][0], Pprev[r][1], Pprev[r][2], Pprev[r][3]};
-                    update_planes(P, set_m, NZ & valid, rc.mk);
-                    transpose4x4(P[0], P[1], P[2], P[3]);
-                    if (rv) out[(size_t)z * plane_stride + (size_t)y * row_stride + xg] = make_uint4(P[0], P[1], P[2], P[3]);
-                }
-                if (any) {
-                    zmin = min(zmin, z);
-                    zmax = max(zmax, z);
-                }
-            }
-
-            // rotate the z pipeline
-#pragma unroll
-            for (int r = 0; r < RPL; r++) {
-                S1pp[r] = S1p[r];
-                S1p[r] = S1c[r];
-                S0p[r] = S0c[r];
-                for (int k = 0; k < 4; k++) Pprev[r][k] = Pcur[r][k];
-                Aprev[r] = Acur[r];
-                NZprev[r] = NZcur[r];
-            }
+        // steps zi = z0-1 .. z1, two per iteration with alternating register sets
+        int zi = L.z0 - 1;
+        for (; zi + 1 <= L.z1; zi += 2) {
+            L.step(zi, c0, c1);
+            L.step(zi + 1, c1, c0);
         }
+        if (zi <= L.z1) L.step(zi, c0, c1);
 
-        // bounding box of this lane's live cells
-        if (colmask) {
+        acc.surv = L.surv;
+        acc.born = L.born;
+        acc.dying = L.dying;
+        // cells that stayed 0 -> 0: every valid cell not counted as dying / born / surviving
+        acc.dead = (uint32_t)__popc(L.vmask) * L.nrows - L.dying - L.born - L.surv;
+        if (L.colmask) {
             int lo, hi;
-            il_extent(colmask, lo, hi);
-            acc.bmin[0] = (int)(xg * 32u) + lo;
-            acc.bmax[0] = (int)(xg * 32u) + hi;
-            acc.bmin[2] = zmin;
-            acc.bmax[2] = zmax;
+            mask_extent<PLANAR>(L.colmask, lo, hi);
+            acc.bmin[0] = (int)(L.xg * 32u) + lo;
+            acc.bmax[0] = (int)(L.xg * 32u) + hi;
+            acc.bmin[2] = L.zmin;
+            acc.bmax[2] = L.zmax;
 #pragma unroll
             for (int r = 0; r < RPL; r++)
-                if (yany[r]) {
-                    acc.bmin[1] = min(acc.bmin[1], y_first + r);
-                    acc.bmax[1] = max(acc.bmax[1], y_first + r);
+                if (L.yany[r]) {
+                    acc.bmin[1] = min(acc.bmin[1], L.y_first + r);
+                    acc.bmax[1] = max(acc.bmax[1], L.y_first + r);
                 }
         }
     }
     block_reduce_stats(acc, &partials[blockIdx.x], red);
+}
+
+// ---------------------------------------------------------------------------
+// planar layout: init and conversion from / to the reference nibble layout
+// ---------------------------------------------------------------------------
+__global__ void k_init_planar(uint4 *cells, GridGeom g, DevInit init, uint32_t M)
+{
+    const uint32_t G = g.pitch / 4u;
+    const size_t n = (size_t)G * g.Y * g.Z;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t gx = (uint32_t)(i % G);
+        const size_t row = i / G;
+        const uint32_t y = (uint32_t)(row % g.Y);
+        const uint32_t z = (uint32_t)(row / g.Y);
+        uint32_t bits = 0u;
+        if (y >= init.box_lo[1] && y < init.box_hi[1] && z >= init.box_lo[2] && z < init.box_hi[2]) {
+            const uint32_t xa = max(32u * gx, init.box_lo[0]);
+            const uint32_t xb = min(min(32u * gx + 32u, init.box_hi[0]), g.X);
+            for (uint32_t x = xa; x < xb; x++)
+                if (init_alive(init, x, y, z)) bits |= 1u << (x - 32u * gx);
+        }
+        const uint32_t m = M & 15u;
+        cells[i] = make_uint4((m & 1u) ? bits : 0u, (m & 2u) ? bits : 0u, (m & 4u) ? bits : 0u, (m & 8u) ? bits : 0u);
+    }
+}
+
+// reference layout (ceil(X/8) words per row) -> planar groups
+__global__ void k_ref_to_planar(const uint32_t *__restrict__ ref, uint4 *__restrict__ planar, GridGeom g)
+{
+    const uint32_t G = g.pitch / 4u, upr = (g.X + 7u) / 8u;
+    const size_t n = (size_t)G * g.Y * g.Z;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t gx = (uint32_t)(i % G);
+        const size_t row = i / G;
+        uint32_t P[4] = {0u, 0u, 0u, 0u};
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t xw = 4u * gx + j;
+            if (xw >= upr) break;
+            const uint32_t w = ref[row * upr + xw];
+            for (uint32_t c = 0; c < 8; c++) {
+                if (8u * xw + c >= g.X) break;  // row padding is never a cell
+                for (uint32_t k = 0; k < 4; k++) P[k] |= ((w >> (4u * c + k)) & 1u) << (8u * j + c);
+            }
+        }
+        planar[i] = make_uint4(P[0], P[1], P[2], P[3]);
+    }
+}
+
+// planar groups -> reference layout
+__global__ void k_planar_to_ref(const uint4 *__restrict__ planar, uint32_t *__restrict__ ref, GridGeom g)
+{
+    const uint32_t G = g.pitch / 4u, upr = (g.X + 7u) / 8u;
+    const size_t n = (size_t)upr * g.Y * g.Z;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t xw = (uint32_t)(i % upr);
+        const size_t row = i / upr;
+        const uint4 q = planar[row * G + xw / 4u];
+        const uint32_t P[4] = {q.x, q.y, q.z, q.w};
+        const uint32_t sh = 8u * (xw % 4u);
+        uint32_t w = 0u;
+        for (uint32_t c = 0; c < 8; c++)
+            for (uint32_t k = 0; k < 4; k++) w |= ((P[k] >> (sh + c)) & 1u) << (4u * c + k);
+        ref[i] = w;
+    }
+}
+
+static uint32_t grid_blocks(size_t n) { return (uint32_t)std::min<size_t>((n + 255) / 256, 65535u * 4u); }
+
+int launch_init_planar(uint32_t *cells, const GridGeom &g, const DevInit &init, uint32_t M, hipStream_t st)
+{
+    const size_t n = (size_t)(g.pitch / 4u) * g.Y * g.Z;
+    hipLaunchKernelGGL(k_init_planar, dim3(grid_blocks(n)), dim3(256), 0, st, reinterpret_cast<uint4 *>(cells), g,
+                       init, M);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+int launch_ref_to_planar(const uint32_t *ref, uint32_t *planar, const GridGeom &g, hipStream_t st)
+{
+    const size_t n = (size_t)(g.pitch / 4u) * g.Y * g.Z;
+    hipLaunchKernelGGL(k_ref_to_planar, dim3(grid_blocks(n)), dim3(256), 0, st, ref,
+                       reinterpret_cast<uint4 *>(planar), g);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
+}
+
+int launch_planar_to_ref(const uint32_t *planar, uint32_t *ref, const GridGeom &g, hipStream_t st)
+{
+    const size_t n = (size_t)((g.X + 7u) / 8u) * g.Y * g.Z;
+    hipLaunchKernelGGL(k_planar_to_ref, dim3(grid_blocks(n)), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(planar), ref, g);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -259,7 +419,11 @@ static uint32_t il_valid_mask(uint32_t ncells)
     return m;
 }
 
-bool stream_supported(const GridGeom &g) { return g.X >= 1 && g.X <= 512 && g.pitch == 4u * ((g.X + 31u) / 32u); }
+bool stream_supported(const GridGeom &g)
+{   // rows of <= 16 groups (one DPP row) and buffers addressable by 32-bit offsets below kOutOfRange / 2
+    return g.X >= 1 && g.X <= 512 && g.pitch == 4u * ((g.X + 31u) / 32u) &&
+           (uint64_t)g.pitch * 4u * g.Y * g.Z < (1ull << 30);
+}
 
 StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t waves)
 {
@@ -275,9 +439,9 @@ StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t wa
     if (tz) {
         p.tz = tz;
     } else {
-        // aim at >= 8 waves per CU on a 256-CU part, with z tiles of >= 8 planes
+        // aim at >= 16 waves per CU on a 256-CU part (4 per SIMD), with z tiles of >= 8 planes
         uint32_t t = 64;
-        while (t > 8 && (uint64_t)p.ytiles * ((g.Z + t - 1) / t) < 2048) t >>= 1;
+        while (t > 8 && (uint64_t)p.ytiles * ((g.Z + t - 1) / t) < 4096) t >>= 1;
         p.tz = std::min(t, g.Z);
     }
     p.ztiles = (g.Z + p.tz - 1) / p.tz;
@@ -286,15 +450,31 @@ StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t wa
     return p;
 }
 
-template <uint32_t FLAGS>
+template <uint32_t FLAGS, bool PLANAR>
 static void launch_flags(const uint4 *in, uint4 *out, const StreamArgs &a, const StreamPlan &p, StatPart *parts,
                          const uint32_t *done, hipStream_t st)
 {
     const dim3 grid(p.blocks), block(64 * p.waves);
     switch (p.rpl) {
-    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1>), grid, block, 0, st, in, out, a, parts, done); break;
-    case 2: hipLaunchKernelGGL((k_step_stream<FLAGS, 2>), grid, block, 0, st, in, out, a, parts, done); break;
-    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 2: hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
+    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
+    }
+}
+
+template <bool PLANAR>
+static void launch_layout(uint32_t flags, const uint4 *i4, uint4 *o4, const StreamArgs &a, const StreamPlan &p,
+                          StatPart *partials, const uint32_t *done, hipStream_t st)
+{
+    switch (flags & 7u) {
+    case 1: launch_flags<1, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 2: launch_flags<2, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 3: launch_flags<3, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 4: launch_flags<4, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 5: launch_flags<5, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 6: launch_flags<6, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    case 7: launch_flags<7, PLANAR>(i4, o4, a, p, partials, done, st); break;
+    default: launch_flags<0, PLANAR>(i4, o4, a, p, partials, done, st); break;  // no neighbourhood: counts stay 0
     }
 }
 
@@ -316,20 +496,12 @@ int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
     a.ytiles = p.ytiles;
     a.total_waves = p.ytiles * p.ztiles;
     const uint32_t rem = g.X % 32u;
-    a.last_mask = rem ? il_valid_mask(rem) : ~0u;
+    a.last_mask = rem == 0 ? ~0u : (g.planar ? ((1u << rem) - 1u) : il_valid_mask(rem));
     a.rc = rule_consts(rule.M, rule.born_lut, rule.surv_lut);
     const uint4 *i4 = reinterpret_cast<const uint4 *>(in);
     uint4 *o4 = reinterpret_cast<uint4 *>(out);
-    switch (rule.flags & 7u) {
-    case 1: launch_flags<1>(i4, o4, a, p, partials, done, st); break;
-    case 2: launch_flags<2>(i4, o4, a, p, partials, done, st); break;
-    case 3: launch_flags<3>(i4, o4, a, p, partials, done, st); break;
-    case 4: launch_flags<4>(i4, o4, a, p, partials, done, st); break;
-    case 5: launch_flags<5>(i4, o4, a, p, partials, done, st); break;
-    case 6: launch_flags<6>(i4, o4, a, p, partials, done, st); break;
-    case 7: launch_flags<7>(i4, o4, a, p, partials, done, st); break;
-    default: launch_flags<0>(i4, o4, a, p, partials, done, st); break;  // no neighbourhood: counts stay 0
-    }
+    if (g.planar) launch_layout<true>(rule.flags, i4, o4, a, p, partials, done, st);
+    else launch_layout<false>(rule.flags, i4, o4, a, p, partials, done, st);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

@@ -1,0 +1,122 @@
+"""Synthetic workloads of the benchmark configurations (seeded, reproducible).
+
+Every random draw goes through the library's splitmix64 stream (cave_amd.Rng),
+and cell initialisation through the reference cell hash, so the same seeds give
+the same simulations on the CPU oracle and on the GPU.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import cave_amd as cv
+
+FLAGS_F, FLAGS_FE, FLAGS_FEV = 1, 3, 7
+
+
+def rule_bits(rng: cv.Rng, flags: int, p: float) -> tuple[int, int]:
+    n = {1: 6, 2: 12, 3: 18, 4: 8, 5: 14, 6: 20, 7: 26}[flags]
+    return rng.rule(n, p), rng.rule(n, p)
+
+
+def single_grid_rule(seed: int = 2) -> tuple[int, int]:
+    """512^3 FEV workload: birth / survival bits over counts 1..26, p = 0.15."""
+    rng = cv.Rng(seed)
+    return rng.rule(26, 0.15), rng.rule(26, 0.15)
+
+
+def search_uniform(n_configs: int = 4096, seeds: int = 32, ticks: int = 20, seed: int = 3) -> np.ndarray:
+    """Batched rule search, uniform size: n_configs rule configs, each a 49^3
+    FE grid, <= 2 birth and <= 2 survival bits from counts 1..18 (uniform over
+    the 171 masks of that set), C uniform in 2..5, `seeds` spawn seeds per
+    config (central 12^3 Bernoulli(0.3), hash seeds 0..seeds-1).  Config-major
+    order: row c * seeds + s."""
+    rng = cv.Rng(seed)
+    vb = cv.enumerate_valid_bitmasks(18, 2)
+    M = np.empty(n_configs, np.uint32)
+    B = np.empty(n_configs, np.uint64)
+    S = np.empty(n_configs, np.uint64)
+    for c in range(n_configs):
+        M[c] = 1 + rng.next() % 4
+        B[c] = vb[rng.next() % len(vb)]
+        S[c] = vb[rng.next() % len(vb)]
+    a = np.zeros(n_configs * seeds, dtype=cv.SIM_DTYPE)
+    a["dims"] = 49
+    a["M"] = np.repeat(M, seeds)
+    a["flags"] = FLAGS_FE
+    a["ticks"] = ticks
+    a["birth"] = np.repeat(B, seeds)
+    a["survival"] = np.repeat(S, seeds)
+    a["spawn"] = 12
+    a["spawn_mode"] = cv.SPAWN_RANDOM
+    a["density"] = 0.3
+    a["seed"] = np.tile(np.arange(seeds, dtype=np.uint32), n_configs)
+    return a
+
+
+def search_uneven(n_configs: int = 2048, seed: int = 4) -> np.ndarray:
+    """Batched rule search, uneven sizes: edge log-uniform in [24, 160],
+    F : FE : FEV = 1 : 2 : 1, C uniform in 2..16, bits p = 0.1, central cube of
+    edge/2 Bernoulli(0.25), tick budget uniform in [20, 200]."""
+    rng = cv.Rng(seed)
+    a = np.zeros(n_configs, dtype=cv.SIM_DTYPE)
+    for c in range(n_configs):
+        edge = int(round(math.exp(math.log(24) + rng.uniform() * (math.log(160) - math.log(24)))))
+        flags = (FLAGS_F, FLAGS_FE, FLAGS_FE, FLAGS_FEV)[rng.next() % 4]
+        M = 1 + rng.next() % 15
+        b, s = rule_bits(rng, flags, 0.1)
+        a[c]["dims"] = edge
+        a[c]["M"] = M
+        a[c]["flags"] = flags
+        a[c]["ticks"] = 20 + rng.next() % 181
+        a[c]["birth"] = b
+        a[c]["survival"] = s
+        a[c]["spawn"] = max(1, edge // 2)
+        a[c]["spawn_mode"] = cv.SPAWN_RANDOM
+        a[c]["density"] = 0.25
+        a[c]["seed"] = c
+    return a
+
+
+def serving_requests(rate_hz: float = 2000.0, duration_s: float = 60.0, seed: int = 5):
+    """Online serving: Poisson arrivals; per request grid edge log-uniform in
+    [32, 192], F / FE / FEV uniform, C in 2..16, bits p = 0.12, tick budget in
+    [20, 500].  Returns (arrival times in seconds, SIM_DTYPE array)."""
+    rng = cv.Rng(seed)
+    t, times, rows = 0.0, [], []
+    while True:
+        t += -math.log(1.0 - rng.uniform()) / rate_hz
+        if t >= duration_s:
+            break
+        times.append(t)
+        edge = int(round(math.exp(math.log(32) + rng.uniform() * (math.log(192) - math.log(32)))))
+        flags = (FLAGS_F, FLAGS_FE, FLAGS_FEV)[rng.next() % 3]
+        M = 1 + rng.next() % 15
+        b, s = rule_bits(rng, flags, 0.12)
+        ticks = 20 + rng.next() % 481
+        rows.append((edge, M, flags, ticks, b, s, len(times) - 1))
+    a = np.zeros(len(rows), dtype=cv.SIM_DTYPE)
+    for i, (edge, M, flags, ticks, b, s, sd) in enumerate(rows):
+        a[i]["dims"] = edge
+        a[i]["M"] = M
+        a[i]["flags"] = flags
+        a[i]["ticks"] = ticks
+        a[i]["birth"] = b
+        a[i]["survival"] = s
+        a[i]["spawn"] = max(1, edge // 2)
+        a[i]["spawn_mode"] = cv.SPAWN_RANDOM
+        a[i]["density"] = 0.25
+        a[i]["seed"] = sd
+    return np.array(times), a
+
+
+def aggregate_by_config(specs: np.ndarray, res: np.ndarray, seeds: int):
+    """RuleAggregator-style stats per config (src/common/ruleAggregator.h:69-90):
+    total / viable / possible / unviable runs and min / max / sum of ticks."""
+    cls = res["classification"].reshape(-1, seeds)
+    ticks = res["ticks_survived"].reshape(-1, seeds).astype(np.int64)
+    return {
+        "viable": (cls == 2).sum(1), "possible": (cls == 1).sum(1), "unviable": (cls == 0).sum(1),
+        "min_ticks": ticks.min(1), "max_ticks": ticks.max(1), "sum_ticks": ticks.sum(1),
+    }

@@ -137,8 +137,9 @@ enum {
 
 enum {
     CAVE_KERNEL_AUTO = 0,
-    CAVE_KERNEL_GATHER = 1, /* one lane per packed word, per-cell gather */
-    CAVE_KERNEL_STREAM = 2  /* bit-sliced z-streaming kernel            */
+    CAVE_KERNEL_GATHER = 1,        /* one lane per packed word, per-cell gather            */
+    CAVE_KERNEL_STREAM = 2,        /* bit-sliced z-streaming kernel, bit-planar cell groups */
+    CAVE_KERNEL_STREAM_NIBBLE = 3  /* the same kernel on the reference nibble layout        */
 };
 
 typedef struct CaveStepSpec {
@@ -163,7 +164,7 @@ int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words);
 int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words);
 /* Replaces the rule (same M / flags constraints as create). */
 int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params);
-/* Which kernel the handle runs (CAVE_KERNEL_GATHER or CAVE_KERNEL_STREAM). */
+/* Which kernel the handle runs (CAVE_KERNEL_GATHER / _STREAM / _STREAM_NIBBLE). */
 int cave_step_kernel(const CaveStep *h);
 int cave_step_destroy(CaveStep *h);
 

@@ -12,8 +12,8 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-KERNELS = [cv.KERNEL_GATHER, cv.KERNEL_STREAM]
-KNAME = {cv.KERNEL_GATHER: "gather", cv.KERNEL_STREAM: "stream"}
+KERNELS = [cv.KERNEL_GATHER, cv.KERNEL_STREAM, cv.KERNEL_STREAM_NIBBLE]
+KNAME = {cv.KERNEL_GATHER: "gather", cv.KERNEL_STREAM: "stream", cv.KERNEL_STREAM_NIBBLE: "stream_nibble"}
 
 
 def stats_row(st):
@@ -106,3 +106,27 @@ def test_multi_tick_run_matches_single_ticks(kernel):
     assert np.array_equal(a.cells(), b.cells())
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
+def test_garbage_row_padding_is_ignored(kernel):
+    # the reference never reads padding nibbles as neighbours; live-valued
+    # garbage in them must not change any result
+    X, Y, Z, M, flags = 37, 6, 5, 3, 7
+    rng = np.random.default_rng(77)
+    import np_model as NP
+    dense = rng.integers(0, M + 1, size=(Z, Y, X)).astype(np.uint8)
+    packed = NP.pack(dense)
+    dirty = packed.copy().reshape(Z, Y, -1)
+    dirty[:, :, -1] |= np.uint32(0x33300000)  # last word: nibbles 5..7 (x = 37..39, padding) set to M
+    dirty = dirty.reshape(-1)
+    birth, surv = 0b1110, 0b111000
+    w0 = ((M - 1) & 0xF) << 60 | birth
+    w1 = flags << 60 | surv
+    with cv.Step((X, Y, Z), M, flags, birth, surv, initial_cells=dirty, kernel=kernel) as h:
+        ref = packed
+        for t in range(3):
+            ref, st_ref = O.step(ref, X, Y, Z, w0, w1)
+            (st,), _ = h.run(1)
+            assert st.as_tuple() == st_ref.as_tuple(), f"tick {t}"
+            assert np.array_equal(h.cells(), ref), f"tick {t}"
