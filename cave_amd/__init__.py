@@ -23,7 +23,7 @@ FACES, EDGES, CORNERS, WRAP = 1, 2, 4, 8
 SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER = 0, 1, 2
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE = 0, 1, 2, 3
 ENGINE_AUTO, ENGINE_PERSISTENT, ENGINE_PLAIN = 0, 1, 2
-SIM_NO_EARLY_EXIT, SIM_FINAL_HASH = 1, 2
+SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH = 1, 2, 4
 
 
 class CaveError(RuntimeError):
@@ -99,7 +99,7 @@ class SearchSpec(C.Structure):
     _fields_ = [("chunk", ChunkConfiguration), ("neighborhood_flags", C.c_uint32),
                 ("max_rule_bits", C.c_uint32), ("min_max_cell_state", C.c_int32),
                 ("max_max_cell_state", C.c_int32), ("max_ticks", C.c_uint32),
-                ("reserved", C.c_uint32), ("output_folder", C.c_char_p), ("file_prefix", C.c_char_p)]
+                ("sim_flags", C.c_uint32), ("output_folder", C.c_char_p), ("file_prefix", C.c_char_p)]
 
 
 class SearchResult(C.Structure):
@@ -385,3 +385,80 @@ def batch_run(sims, engine=ENGINE_AUTO, per_tick=False, block_threads=0):
     if per_tick:
         rows = [[pt[i * stride + t] for t in range(res[i].ticks_run)] for i in range(n)]
     return [res[i] for i in range(n)], rows, opt.device_ms
+
+
+# ---- reference-compatible search chunk (SearchSystem) ----
+class Search:
+    """One --mode search chunk: every (maxCS, neighbourhood subset) slot and
+    every (birth, survival) permutation of its partition, evaluated in batches
+    on the GPU, streamed to <folder><prefix>_{viable,possible,unviable}.jsonl."""
+
+    def __init__(self, grid, flags, max_rule_bits, min_cs, max_cs, ticks, spawn=11, spawn_mode=SPAWN_FILLED,
+                 density_index=7, seed_index=0, partition_index=0, partition_count=1, output_folder=None,
+                 file_prefix="", sim_flags=0):
+        g = grid if isinstance(grid, (tuple, list)) else (grid, grid, grid)
+        sp = spawn if isinstance(spawn, (tuple, list)) else (spawn, spawn, spawn)
+        s = SearchSpec()
+        c = s.chunk
+        c.shapeId = 0
+        c.gridDimensionX, c.gridDimensionY, c.gridDimensionZ = g
+        c.spawnAreaDimensionX, c.spawnAreaDimensionY, c.spawnAreaDimensionZ = [min(a, b) for a, b in zip(sp, g)]
+        c.spawnRegionShapeId = spawn_mode
+        c.spawnDensityIndex = density_index
+        c.runSeedIndex = seed_index
+        c.partitionIndex = partition_index
+        c.partitionCount = partition_count
+        s.neighborhood_flags = flags
+        s.max_rule_bits = max_rule_bits
+        s.min_max_cell_state = min_cs
+        s.max_max_cell_state = max_cs
+        s.max_ticks = ticks
+        s.sim_flags = sim_flags
+        self._folder = output_folder.encode() if output_folder else None
+        self._prefix = file_prefix.encode()
+        s.output_folder = self._folder
+        s.file_prefix = self._prefix
+        h = C.c_void_p()
+        check(lib().cave_search_create(C.byref(s), C.byref(h)))
+        self._h = h
+
+    def step(self, max_evals: int = 0) -> None:
+        check(lib().cave_search_step(self._h, max_evals))
+
+    def run(self) -> None:
+        self.step(0)
+
+    @property
+    def running(self) -> bool:
+        return bool(lib().cave_search_is_running(self._h))
+
+    def progress(self) -> dict:
+        p = SearchProgress()
+        check(lib().cave_search_progress(self._h, C.byref(p)))
+        return {n: getattr(p, n) for n, _ in SearchProgress._fields_}
+
+    def drain(self, cap: int = 1 << 16):
+        out = []
+        n = C.c_size_t(0)
+        while True:
+            buf = (SearchResult * cap)()  # fresh buffer: the returned structs view its memory
+            check(lib().cave_search_drain(self._h, buf, cap, C.byref(n)))
+            out.extend(buf[i] for i in range(n.value))
+            if n.value < cap:
+                return out
+
+    def summary(self) -> str:
+        buf = C.create_string_buffer(4096)
+        check(lib().cave_search_write_summary(self._h, buf, 4096))
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cave_search_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

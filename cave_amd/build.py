@@ -35,11 +35,12 @@ LIB_SOURCES = [
     "api_step.cpp",
     "api_batch.cpp",
     "api_pending.cpp",
+    "api_search.cpp",
     "batch.hip",
     "step_gather.hip",
     "step_stream.hip",
 ]
-CLI_SOURCES: list[str] = []
+CLI_SOURCES: list[str] = ["cli.cpp"]
 
 
 def _headers_mtime() -> float:

@@ -284,7 +284,9 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
                                : launch_gather(in, out, g, rule, dparts.as<StatPart>(), st, done);
             if (rc) return rc;
             const uint32_t nparts = use_stream[i] ? plans[i].blocks : gather_blocks(g);
-            if ((rc = launch_plain_manager(dparts.as<StatPart>(), nparts, state, s.ticks, s.flags, pt, st))) return rc;
+            if ((rc = launch_plain_manager(dparts.as<StatPart>(), nparts, state, s.ticks, s.flags, pt,
+                                           g.X * g.Y * g.Z, st)))
+                return rc;
         }
         if (s.flags & CAVE_SIM_FINAL_HASH) {
             CAVE_HIP(hipMemcpyAsync(&hs[i], state, sizeof(PlainState), hipMemcpyDeviceToHost, st));

@@ -229,7 +229,10 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
                 info.totalNumberOfCellsThatWereBorn += p.born;
                 info.finalAliveCount = p.surv;
                 info.maxAliveCount = max(info.maxAliveCount, p.surv);
-                if (p.surv == 0u && p.born == 0u) info.status = 1u;
+                const bool died = (j.sim_flags & CAVE_SIM_LEGACY_DEATH)
+                                      ? (uint64_t)p.dead == (uint64_t)j.X * j.Y * j.Z
+                                      : (p.surv == 0u && p.born == 0u);
+                if (died) info.status = 1u;
                 else if (info.numberOfSimulationTicksSurvived >= j.ticks) info.status = 2u;
                 else info.status = 0u;
                 last.snap.numberOfDeadCells = p.dead;

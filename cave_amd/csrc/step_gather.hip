@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(256) k_reduce_ticks(const StatPart *__restrict
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_plain_manager(const StatPart *__restrict__ partials, uint32_t nparts,
                                                        PlainState *__restrict__ s, uint32_t ticks,
-                                                       uint32_t sim_flags, CaveTickStats *__restrict__ per_tick)
+                                                       uint32_t sim_flags, CaveTickStats *__restrict__ per_tick,
+                                                       uint32_t total_cells)
 {
     __shared__ int32_t red[10 * 4];
     __shared__ StatPart tot;
@@ -190,7 +191,9 @@ __global__ void __launch_bounds__(256) k_plain_manager(const StatPart *__restric
         info.totalNumberOfCellsThatWereBorn += tot.born;
         info.finalAliveCount = tot.surv;
         info.maxAliveCount = max(info.maxAliveCount, tot.surv);
-        if (tot.surv == 0u && tot.born == 0u) info.status = 1u;
+        const bool died = (sim_flags & CAVE_SIM_LEGACY_DEATH) ? tot.dead == total_cells
+                                                              : (tot.surv == 0u && tot.born == 0u);
+        if (died) info.status = 1u;
         else if (info.numberOfSimulationTicksSurvived >= ticks) info.status = 2u;
         else info.status = 0u;
         CaveTickStats l;
@@ -213,9 +216,10 @@ __global__ void __launch_bounds__(256) k_plain_manager(const StatPart *__restric
 }
 
 int launch_plain_manager(const StatPart *partials, uint32_t nparts, PlainState *state, uint32_t ticks,
-                         uint32_t sim_flags, CaveTickStats *per_tick, hipStream_t st)
+                         uint32_t sim_flags, CaveTickStats *per_tick, uint32_t total_cells, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_plain_manager, dim3(1), dim3(256), 0, st, partials, nparts, state, ticks, sim_flags, per_tick);
+    hipLaunchKernelGGL(k_plain_manager, dim3(1), dim3(256), 0, st, partials, nparts, state, ticks, sim_flags, per_tick,
+                       total_cells);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

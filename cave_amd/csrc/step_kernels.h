@@ -63,6 +63,6 @@ struct PlainState {
     CaveTickStats last;
 };
 int launch_plain_manager(const StatPart *partials, uint32_t nparts, PlainState *state, uint32_t ticks,
-                         uint32_t sim_flags, CaveTickStats *per_tick, hipStream_t st);
+                         uint32_t sim_flags, CaveTickStats *per_tick, uint32_t total_cells, hipStream_t st);
 
 }  // namespace cave

