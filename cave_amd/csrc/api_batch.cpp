@@ -127,7 +127,7 @@ static uint32_t classify(const CaveGridInfo &info, uint32_t M)
 int batch_persistent(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt,
                      hipStream_t st)
 {
-    uint32_t threads = opt && opt->block_threads ? opt->block_threads : 512u;
+    uint32_t threads = opt && opt->block_threads ? opt->block_threads : 1024u;
     threads = std::max(64u, std::min(1024u, (threads + 63u) / 64u * 64u));
     const uint32_t lds_budget = kLdsBytesPerCU - kStaticLds;
 
@@ -137,6 +137,7 @@ int batch_persistent(const CaveSimSpec *specs, size_t n, CaveSimResult *results,
     uint64_t final_words = 0;
     for (size_t i = 0; i < n; i++) {
         jobs[i] = make_job(specs[i], (uint32_t)i);
+        jobs[i].zchunks = job_zchunks(jobs[i].X, jobs[i].Y, jobs[i].Z, threads);
         const uint64_t w = job_words(jobs[i].X, jobs[i].Y, jobs[i].Z);
         if (w * 4u <= lds_budget) {
             jobs[i].tier = 0;

@@ -22,6 +22,7 @@ struct DevJob {
     uint32_t tier;         // 0: cells resident in LDS, 1: in the global arena
     uint32_t result_slot;  // index into results / per-tick rows
     uint32_t final_slot;   // +1-based index of a final-cells staging area (0 = none)
+    uint32_t zchunks;      // z-chunks per column per tick (work items = columns x zchunks)
     DevInit init;
 };
 
@@ -47,11 +48,24 @@ struct BatchArgs {
     uint32_t lds_words;          // dynamic LDS words available
 };
 
-// Words of cell storage a job needs: 4 state planes + 2 live planes of
-// ceil(X/32) words per row.
+// Words of cell storage a job needs: 4 state planes of ceil(X/32) words per
+// row, 2 live planes with a one-row / one-plane zero guard band, and one
+// padding word past the end (read, masked, by the last x-neighbour load).
 inline uint64_t job_words(uint32_t X, uint32_t Y, uint32_t Z)
 {
-    return 6ull * ((X + 31u) / 32u) * (uint64_t)Y * Z;
+    const uint64_t W = (X + 31u) / 32u;
+    return 4ull * W * Y * Z + 2ull * W * (Y + 2ull) * (Z + 2ull) + 2ull;
+}
+
+// Work split of one tick: columns (x-word, y) times z-chunks, about one item
+// per thread of the block when the grid allows it.
+inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads)
+{
+    const uint32_t cols = ((X + 31u) / 32u) * Y;
+    uint32_t z = threads / (cols ? cols : 1u);
+    if (z < 1u) z = 1u;
+    if (z > Z) z = Z;
+    return z;
 }
 
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
