@@ -51,12 +51,17 @@ def workload_rule():
     return birth, surv
 
 
-def load_traffic(kernel_name: str):
-    path = os.path.join(ROOT, "profiles", f"{kernel_name}_traffic.json")
+def load_traffic(summary: str):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprof
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    return None
 
 
 def run_step_bench(steps: int, warmup: int):
@@ -156,6 +161,22 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     return out
 
 
+def copy_calibration(mib: int, copies: int):
+    """K plain device-to-device copies of a MIB-MiB buffer: each reads and
+    writes exactly MIB MiB, the yardstick for the HBM byte counters."""
+    import torch
+
+    n = mib << 20
+    src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(copies):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n, "bytes_written_per_copy": n,
+                                           "copies": copies}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,11 +185,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-configs", type=int, default=4096)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run K device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the whole grid lives on one MI355X and "
               "search/serving configs are independent (no multi-GPU path is built).", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration, a.steps)
+        return
 
     import cave_amd as cv
 
@@ -178,7 +206,7 @@ def main():
     value = cells * a.steps / wall
     launch_s = dev_ms / 1e3 / a.steps
     achieved = cells * 1.0 / launch_s / 1e9  # 0.5 B read + 0.5 B write per cell-update
-    traffic = load_traffic(f"step_{kernel}_512_FEV")
+    traffic = load_traffic("step_stream_512_FEV_C8.md") if kernel == "stream" else None
     out = {
         "metric": METRIC,
         "value": value,
