@@ -9,8 +9,9 @@ Bernoulli(0.2) initial cells (cell-hash seed 42).  Inputs are resident in HBM
 before the timed region.  `value` is cell-updates/s over the K timed ticks.
 
 The same JSON line also carries the rule-search rate at 49^3 x 20 ticks
-(4096 configs x 32 seeds, FE, C in 2..5) and the host-CPU baseline, timed on
-a bounded sample with the CPU oracle.
+(4096 configs x 32 seeds, FE, C in 2..5), the uneven-size autotuned search
+(2048 configs, edge 24..160) with its per-bucket achieved fractions, and the
+host-CPU baselines, each timed on a bounded sample with the CPU oracle.
 """
 from __future__ import annotations
 
@@ -27,6 +28,11 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/sec + % HBM roofline; rule-configs/sec at 49^3×20 ticks; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 GRID = 512
+
+
+def log(msg: str):
+    """Progress line on stderr (the JSON result line is the only stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_info():
@@ -131,6 +137,7 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
         "classes": {"viable": int(agg["viable"].sum()), "possible": int(agg["possible"].sum()),
                     "unviable": int(agg["unviable"].sum())},
     }
+    log(f"  persistent engine {dev_ms:.2f} ms; plain engine on {plain_configs} configs")
     sub = specs[: plain_configs * seeds]
     pres, pms = cv.batch_run_np(sub, engine=cv.ENGINE_PLAIN)
     assert np.array_equal(pres["ticks_run"], res["ticks_run"][: len(sub)]), "plain / persistent disagree"
@@ -158,6 +165,89 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
                                "cell_updates_per_s": cpu_updates / dt, "cores": 1, "kind": "port",
                                "sample": f"first {done} sims ({done / seeds:.2f} configs) of the same list, "
                                          "scalar oracle, 1 thread"}
+    return out
+
+
+EDGE_BUCKETS = ["24-31", "32-47", "48-63", "64-95", "96-127", "128-160"]
+FLAG_NAMES = {1: "F", 3: "FE", 7: "FEV"}
+
+
+def run_uneven_bench(n_configs=2048, cpu=True):
+    """Batched rule search with uneven sizes: 2048 configs, edge log-uniform
+    in [24, 160], F : FE : FEV = 1 : 2 : 1, C in 2..16, bits p = 0.1, ticks in
+    [20, 200], central cube of edge/2 Bernoulli(0.25).  The autotuner runs in
+    the untimed first call; the timed call reports per-(edge bucket,
+    neighbourhood) block time and achieved fraction of the block's share of
+    the HBM roofline (1 B per cell-update)."""
+    import numpy as np
+
+    import cave_amd as cv
+    from cave_amd import workloads as W
+
+    specs = W.search_uneven(n_configs)
+    cells = specs["dims"].astype(np.int64).prod(1)
+    t0 = time.perf_counter()
+    cv.batch_run_np(specs)  # autotuner calibration + warm-up
+    tune_s = time.perf_counter() - t0
+    log(f"  autotuner + warm-up call {tune_s:.2f} s")
+    t0 = time.perf_counter()
+    res, dev_ms, tm, threads = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
+    wall = time.perf_counter() - t0
+    _, cus, _ = cv.device_info()
+    resident = cus * max(1, 1024 // threads)
+    block_share = HBM_PEAK_GBS * 1e9 / resident  # B/s of HBM per resident block
+    ups = res["ticks_run"].astype(np.int64) * cells
+    edge = np.cbrt(cells.astype(np.float64))
+    bidx = np.digitize(edge, [32, 48, 64, 96, 128])
+    block_s = (tm["end_us"] - tm["start_us"]) * tm["blocks"] / 1e6
+    buckets = []
+    for b in range(6):
+        for fl in (1, 3, 7):
+            m = (bidx == b) & (specs["flags"] == fl)
+            if not m.any():
+                continue
+            rate = ups[m].sum() / block_s[m].sum()
+            buckets.append({"edge": EDGE_BUCKETS[b], "neighborhood": FLAG_NAMES[fl], "configs": int(m.sum()),
+                            "cell_updates": int(ups[m].sum()), "block_ms": float(block_s[m].sum() * 1e3),
+                            "split_configs": int((tm["blocks"][m] > 1).sum()),
+                            "items_per_thread": int(np.bincount(tm["items_per_thread"][m]).argmax()),
+                            "cell_updates_per_block_s": rate, "achieved_frac": rate * 1.0 / block_share})
+    out = {
+        "metric": "rule-configs/sec, uneven sizes (edge 24..160, F/FE/FEV, C 2..16, 20..200 ticks)",
+        "value": n_configs / (dev_ms / 1e3), "value_wall": n_configs / wall, "unit": "configs/s",
+        "configs": n_configs, "device_ms": dev_ms, "cell_updates": int(ups.sum()),
+        "cell_updates_per_s": ups.sum() / (dev_ms / 1e3),
+        "roofline_frac": ups.sum() * 1.0 / (dev_ms / 1e3) / (HBM_PEAK_GBS * 1e9),
+        "block_threads": int(threads), "tuning_s": tune_s,
+        "split_configs": int((tm["blocks"] > 1).sum()),
+        "last_finish_ms": float(tm["end_us"].max() / 1e3), "p50_finish_ms": float(np.percentile(tm["end_us"], 50) / 1e3),
+        "buckets": buckets,
+    }
+    log(f"  timed call {dev_ms:.2f} ms")
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        order = np.argsort(ups, kind="stable")  # cheapest first: many whole simulations in the sample
+        t0, done, cpu_ups = time.perf_counter(), 0, 0
+        for i in order:
+            if time.perf_counter() - t0 > 10.0:
+                break
+            sp = specs[i]
+            X, Y, Z = (int(v) for v in sp["dims"])
+            init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
+                                float(sp["density"]), int(sp["seed"]))
+            t, info, _, _ = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
+                                       int(sp["survival"]), int(sp["ticks"]), init)
+            assert t == res["ticks_run"][i], "CPU oracle / GPU disagree"
+            cpu_ups += t * X * Y * Z
+            done += 1
+        dt = time.perf_counter() - t0
+        cpu_rate = cpu_ups / dt
+        out["cpu_baseline"] = {"value": cpu_rate / (ups.sum() / n_configs), "unit": "configs/s",
+                               "cell_updates_per_s": cpu_rate, "cores": 1, "kind": "port",
+                               "sample": f"{done} cheapest simulations of the list, scalar oracle, 1 thread; "
+                                         "configs/s = CPU cell-updates/s / mean executed cell-updates per config"}
     return out
 
 
@@ -201,6 +291,7 @@ def main():
     import cave_amd as cv
 
     dev, cus, mem = cv.device_info()
+    log(f"single-grid step: {a.steps} ticks of 512^3 FEV")
     wall, dev_ms, kernel = run_step_bench(a.steps, a.warmup)
     cells = GRID ** 3
     value = cells * a.steps / wall
@@ -230,11 +321,16 @@ def main():
     }
     model, ncpu = cpu_info()
     if not a.no_cpu:
+        log("CPU oracle on the single-grid workload")
         cb = cpu_step_baseline()
         cb["host"] = f"{model} ({ncpu} logical CPUs visible)"
         out["cpu_baseline"] = cb
     if not a.no_search:
+        log(f"rule search: {a.search_configs} configs x 32 seeds at 49^3")
         out["search"] = run_search_bench(a.search_configs, cpu=not a.no_cpu)
+        log("rule search, uneven sizes: 2048 configs")
+        out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
+    log("done")
     print(json.dumps(out), flush=True)
 
 

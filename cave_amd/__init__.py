@@ -92,7 +92,8 @@ class SimResult(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("engine", C.c_uint32), ("block_threads", C.c_uint32),
                 ("per_tick", C.POINTER(TickStats)), ("per_tick_stride", C.c_uint32),
-                ("reserved", C.c_uint32), ("device_ms", C.c_double)]
+                ("max_parts", C.c_uint32), ("device_ms", C.c_double),
+                ("timing", C.c_void_p)]
 
 
 class SearchSpec(C.Structure):
@@ -144,6 +145,7 @@ _SIGS = {
     "cave_step_kernel": (C.c_int, [_vp]),
     "cave_step_destroy": (C.c_int, [_vp]),
     "cave_batch_run": (C.c_int, [C.POINTER(SimSpec), _sz, C.POINTER(SimResult), C.POINTER(BatchOptions)]),
+    "cave_batch_tuning_report": (C.c_int, [C.c_char_p, _sz]),
     "cave_search_create": (C.c_int, [C.POINTER(SearchSpec), C.POINTER(_vp)]),
     "cave_search_step": (C.c_int, [_vp, _u64]),
     "cave_search_is_running": (C.c_int, [_vp]),
@@ -352,19 +354,42 @@ RESULT_DTYPE = np.dtype({
     "itemsize": 88})
 
 
-def batch_run_np(specs: np.ndarray, engine=ENGINE_AUTO, block_threads=0):
-    """Evaluates a SIM_DTYPE array; returns (RESULT_DTYPE array, device ms)."""
+TIMING_DTYPE = np.dtype([("start_us", "<f8"), ("end_us", "<f8"), ("blocks", "<u4"), ("items_per_thread", "<u4")])
+
+
+def batch_run_np(specs: np.ndarray, engine=ENGINE_AUTO, block_threads=0, max_parts=0, timing=False):
+    """Evaluates a SIM_DTYPE array; returns (RESULT_DTYPE array, device ms),
+    plus a TIMING_DTYPE array (device start / end microseconds, blocks, items
+    per thread per simulation) and the block size used when timing is set
+    (persistent engine)."""
     specs = np.ascontiguousarray(specs, dtype=SIM_DTYPE)
     res = np.zeros(len(specs), dtype=RESULT_DTYPE)
     opt = BatchOptions()
     opt.engine = engine
     opt.block_threads = block_threads
+    opt.max_parts = max_parts
+    tm = None
+    if timing:
+        tm = np.zeros(len(specs), dtype=TIMING_DTYPE)
+        opt.timing = tm.ctypes.data
     check(lib().cave_batch_run(specs.ctypes.data_as(C.POINTER(SimSpec)), len(specs),
                                res.ctypes.data_as(C.POINTER(SimResult)), C.byref(opt)))
+    if timing:
+        return res, opt.device_ms, tm, opt.block_threads
     return res, opt.device_ms
 
 
-def batch_run(sims, engine=ENGINE_AUTO, per_tick=False, block_threads=0):
+def tuning_report():
+    """The persistent engine's autotuner table (list of dicts) measured so far
+    in this process."""
+    import json
+
+    buf = C.create_string_buffer(1 << 20)
+    check(lib().cave_batch_tuning_report(buf, len(buf)))
+    return json.loads(buf.value.decode())
+
+
+def batch_run(sims, engine=ENGINE_AUTO, per_tick=False, block_threads=0, max_parts=0):
     """Evaluates every Sim; returns (results, per-tick stats rows or None, device ms)."""
     n = len(sims)
     specs = (SimSpec * max(n, 1))()
@@ -374,6 +399,7 @@ def batch_run(sims, engine=ENGINE_AUTO, per_tick=False, block_threads=0):
     opt = BatchOptions()
     opt.engine = engine
     opt.block_threads = block_threads
+    opt.max_parts = max_parts
     stride = max((s.ticks for s in sims), default=1)
     pt = None
     if per_tick:

@@ -36,6 +36,7 @@ LIB_SOURCES = [
     "api_batch.cpp",
     "api_pending.cpp",
     "api_search.cpp",
+    "batch_plan.cpp",
     "batch.hip",
     "step_gather.hip",
     "step_stream.hip",

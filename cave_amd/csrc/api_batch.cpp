@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "batch_plan.h"
 #include "common.h"
 #include "host_util.h"
 #include "step_kernels.h"
@@ -21,9 +22,6 @@
 namespace cave {
 
 namespace {
-
-constexpr uint32_t kLdsBytesPerCU = 160u * 1024u;
-constexpr uint32_t kStaticLds = 1024u;  // red[] + scalars of k_batch
 
 struct DevBuf {
     void *p = nullptr;
@@ -47,17 +45,6 @@ int dev_alloc(DevBuf &b, size_t bytes)
     return CAVE_OK;
 }
 
-int cu_count()
-{
-    static int cus = 0;
-    if (!cus) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 256;
-        cus = p.multiProcessorCount;
-    }
-    return cus;
-}
-
 }  // namespace
 
 int validate_sim(const CaveSimSpec &s)
@@ -72,24 +59,6 @@ int validate_sim(const CaveSimSpec &s)
     if (s.ticks == 0) return fail(CAVE_E_INVALID, "ticks must be >= 1");
     if (s.spawn_mode > CAVE_SPAWN_FILLED) return fail(CAVE_E_INVALID, "spawn_mode must be random or filled");
     return CAVE_OK;
-}
-
-DevJob make_job(const CaveSimSpec &s, uint32_t slot)
-{
-    DevJob j{};
-    j.X = s.dims[0];
-    j.Y = s.dims[1];
-    j.Z = s.dims[2];
-    j.flags = s.neighborhood_flags & 7u;
-    j.M = s.max_cell_state;
-    const DevRule r = make_rule(s.max_cell_state, s.neighborhood_flags, s.birth, s.survival);
-    j.born_lut = r.born_lut;
-    j.surv_lut = r.surv_lut;
-    j.ticks = s.ticks;
-    j.sim_flags = s.flags;
-    j.result_slot = slot;
-    j.init = make_init(s.dims, s.spawn, s.spawn_mode, s.density, s.seed);
-    return j;
 }
 
 uint64_t fnv_ref_layout(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z)
@@ -127,107 +96,14 @@ static uint32_t classify(const CaveGridInfo &info, uint32_t M)
 int batch_persistent(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt,
                      hipStream_t st)
 {
-    uint32_t threads = opt && opt->block_threads ? opt->block_threads : 1024u;
-    threads = std::max(64u, std::min(1024u, (threads + 63u) / 64u * 64u));
-    const uint32_t lds_budget = kLdsBytesPerCU - kStaticLds;
-
-    std::vector<DevJob> jobs(n);
-    uint64_t lds_words = 0, arena_words = 0;
-    std::vector<uint64_t> final_offsets;
-    uint64_t final_words = 0;
-    for (size_t i = 0; i < n; i++) {
-        jobs[i] = make_job(specs[i], (uint32_t)i);
-        jobs[i].zchunks = job_zchunks(jobs[i].X, jobs[i].Y, jobs[i].Z, threads);
-        const uint64_t w = job_words(jobs[i].X, jobs[i].Y, jobs[i].Z);
-        if (w * 4u <= lds_budget) {
-            jobs[i].tier = 0;
-            lds_words = std::max(lds_words, w);
-        } else {
-            jobs[i].tier = 1;
-            arena_words = std::max(arena_words, w);
-        }
-        if (specs[i].flags & CAVE_SIM_FINAL_HASH) {
-            final_offsets.push_back(final_words);
-            jobs[i].final_slot = (uint32_t)final_offsets.size();
-            final_words += cave_cell_words(jobs[i].X, jobs[i].Y, jobs[i].Z);
-        }
-    }
-    // longest jobs first: the shared queue then balances the tail
-    std::stable_sort(jobs.begin(), jobs.end(), [](const DevJob &a, const DevJob &b) {
-        return (uint64_t)a.X * a.Y * a.Z * a.ticks > (uint64_t)b.X * b.Y * b.Z * b.ticks;
-    });
-
-    const uint32_t lds_bytes = (uint32_t)(lds_words * 4u);
-    uint32_t per_cu = std::max(1u, 2048u / threads);
-    if (lds_bytes) per_cu = std::min(per_cu, std::max(1u, kLdsBytesPerCU / (lds_bytes + kStaticLds)));
-    uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)cu_count() * per_cu, n);
-    blocks = std::max(1u, blocks);
-
-    DevBuf djobs, dnext, dres, dpt, darena, dfinal, dfoff;
-    int rc;
-    if ((rc = dev_alloc(djobs, sizeof(DevJob) * n))) return rc;
-    if ((rc = dev_alloc(dnext, sizeof(uint32_t)))) return rc;
-    if ((rc = dev_alloc(dres, sizeof(DevResult) * n))) return rc;
-    const uint32_t stride = opt ? opt->per_tick_stride : 0u;
-    const bool want_pt = opt && opt->per_tick && stride;
-    if (want_pt && (rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
-    if (arena_words && (rc = dev_alloc(darena, sizeof(uint32_t) * arena_words * blocks))) return rc;
-    if (final_words) {
-        if ((rc = dev_alloc(dfinal, sizeof(uint32_t) * final_words))) return rc;
-        if ((rc = dev_alloc(dfoff, sizeof(uint64_t) * final_offsets.size()))) return rc;
-        CAVE_HIP(hipMemcpyAsync(dfoff.p, final_offsets.data(), sizeof(uint64_t) * final_offsets.size(),
-                                hipMemcpyHostToDevice, st));
-    }
-    CAVE_HIP(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(DevJob) * n, hipMemcpyHostToDevice, st));
-    CAVE_HIP(hipMemsetAsync(dnext.p, 0, sizeof(uint32_t), st));
-
-    BatchArgs a{};
-    a.jobs = djobs.as<DevJob>();
-    a.njobs = (uint32_t)n;
-    a.next_job = dnext.as<uint32_t>();
-    a.results = dres.as<DevResult>();
-    a.per_tick = want_pt ? dpt.as<CaveTickStats>() : nullptr;
-    a.per_tick_stride = stride;
-    a.arena = darena.as<uint32_t>();
-    a.arena_words = arena_words;
-    a.final_cells = dfinal.as<uint32_t>();
-    a.final_offsets = dfoff.as<uint64_t>();
-    a.lds_words = (uint32_t)lds_words;
-
-    hipEvent_t e0, e1;
-    CAVE_HIP(hipEventCreate(&e0));
-    CAVE_HIP(hipEventCreate(&e1));
-    CAVE_HIP(hipEventRecord(e0, st));
-    rc = launch_batch(a, blocks, threads, lds_bytes, st);
+    BatchKnobs k;
+    k.threads = opt ? opt->block_threads : 0u;
+    k.max_parts = opt ? opt->max_parts : 0u;
+    BatchPlan plan;
+    int rc = plan_batch(specs, n, k, plan);
     if (rc) return rc;
-    CAVE_HIP(hipEventRecord(e1, st));
-    std::vector<DevResult> hres(n);
-    CAVE_HIP(hipMemcpyAsync(hres.data(), dres.p, sizeof(DevResult) * n, hipMemcpyDeviceToHost, st));
-    if (want_pt)
-        CAVE_HIP(hipMemcpyAsync(opt->per_tick, dpt.p, sizeof(CaveTickStats) * n * stride, hipMemcpyDeviceToHost, st));
-    std::vector<uint32_t> hfinal(final_words);
-    if (final_words)
-        CAVE_HIP(hipMemcpyAsync(hfinal.data(), dfinal.p, sizeof(uint32_t) * final_words, hipMemcpyDeviceToHost, st));
-    CAVE_HIP(hipStreamSynchronize(st));
-    float ms = 0.f;
-    CAVE_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (opt) opt->device_ms = ms;
-
-    for (size_t i = 0; i < n; i++) {
-        CaveSimResult &r = results[i];
-        std::memset(&r, 0, sizeof r);
-        r.info = hres[i].info;
-        r.classification = hres[i].classification;
-        r.ticks_run = hres[i].ticks_run;
-        r.last = hres[i].last;
-    }
-    for (const DevJob &j : jobs)
-        if (j.final_slot)
-            results[j.result_slot].final_hash =
-                fnv_ref_layout(hfinal.data() + final_offsets[j.final_slot - 1], j.X, j.Y, j.Z);
-    return CAVE_OK;
+    if (opt) opt->block_threads = plan.threads;
+    return run_plan(plan, n, results, opt, st);
 }
 
 int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
@@ -329,9 +205,20 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
     return CAVE_OK;
 }
 
+std::string tuning_report();
+
 }  // namespace cave
 
 using namespace cave;
+
+extern "C" int cave_batch_tuning_report(char *buf, size_t cap)
+{
+    if (!buf || cap == 0) return fail(CAVE_E_INVALID, "null buffer");
+    const std::string s = tuning_report();
+    if (s.size() + 1 > cap) return fail(CAVE_E_INVALID, "buffer too small for the tuning report");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return CAVE_OK;
+}
 
 extern "C" int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *results,
                               CaveBatchOptions *opt)

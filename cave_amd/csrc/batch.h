@@ -19,11 +19,33 @@ struct DevJob {
     uint32_t born_lut, surv_lut;
     uint32_t ticks;        // tick budget
     uint32_t sim_flags;    // CAVE_SIM_*
-    uint32_t tier;         // 0: cells resident in LDS, 1: in the global arena
+    uint32_t tier;         // kTierLds, kTierBlockArena or kTierOwnRegion
     uint32_t result_slot;  // index into results / per-tick rows
     uint32_t final_slot;   // +1-based index of a final-cells staging area (0 = none)
-    uint32_t zchunks;      // z-chunks per column per tick (work items = columns x zchunks)
+    uint32_t zchunks;      // z-chunks per column per tick over all parts (a multiple of parts)
+    uint32_t parts;        // blocks that share the job (1 = one block owns it)
+    uint32_t part;         // this queue entry's part: z-chunks [part*c, (part+1)*c), c = zchunks/parts
+    uint32_t sync_slot;    // JobSync record of a multi-part job
+    uint32_t pad_;
+    uint64_t region;       // word offset of the job's own cells in BatchArgs::job_arena (kTierOwnRegion)
     DevInit init;
+};
+
+// Where a job's cells live while it runs.
+constexpr uint32_t kTierLds = 0;         // the block's LDS, for all of its ticks
+constexpr uint32_t kTierBlockArena = 1;  // the claiming block's private region of the global arena
+constexpr uint32_t kTierOwnRegion = 2;   // a region of the job's own (multi-part jobs, served requests)
+
+// Cross-block rendezvous of the parts of one job.  Tick t uses counter slot
+// t % 3: parts add their block partials, arrive, wait for all parts, then read
+// the job totals.  Zeroed before the launch.
+struct JobSync {
+    uint32_t arrive;        // arrivals over all phases (init = phase 0, tick t = phase t + 1)
+    uint32_t abort;         // set when a wait timed out: every part leaves the job
+    uint32_t pad_[2];
+    uint32_t sum[3][4];     // dying, born, surviving
+    int32_t nmin[3][4];     // 0x7fffffff - bbox min (max-combined, 0 = empty)
+    int32_t xmax1[3][4];    // bbox max + 1 (max-combined, 0 = empty)
 };
 
 // Result record written by the kernel (host converts to CaveSimResult).
@@ -32,6 +54,9 @@ struct DevResult {
     uint32_t classification;
     uint32_t ticks_run;
     CaveTickStats last;
+    uint64_t t_start, t_end;  // device wall clock at claim / at completion
+    uint32_t err;             // 1: a part rendezvous timed out
+    uint32_t pad_;
 };
 
 struct BatchArgs {
@@ -46,6 +71,9 @@ struct BatchArgs {
     uint32_t *final_cells;       // staging for final states (reference layout)
     const uint64_t *final_offsets; // word offset per final slot
     uint32_t lds_words;          // dynamic LDS words available
+    uint32_t *job_arena;         // kTierOwnRegion cells, DevJob::region words in
+    JobSync *sync;               // one record per multi-part job
+    uint64_t wait_limit;         // device wall-clock ticks a part waits for the others
 };
 
 // Words of cell storage a job needs: 4 state planes of ceil(X/32) words per
@@ -68,6 +96,15 @@ inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads
     return z;
 }
 
+// z-chunks of one part of a job split over `parts` blocks: about one item per
+// thread of the block, times the number of parts.
+inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads, uint32_t parts)
+{
+    const uint32_t zp = (Z + parts - 1u) / parts;
+    return job_zchunks(X, Y, zp, threads) * parts;
+}
+
+const void *batch_kernel();
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
 
 }  // namespace cave

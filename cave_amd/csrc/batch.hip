@@ -33,10 +33,13 @@ struct JobGeo {
     uint32_t aplane;     // W * (Y+2) * (Z+2): one guarded live plane
     uint32_t aslab;      // W * (Y+2): words per z plane of a live plane
     uint32_t last_mask;  // valid cells of the last word of a row
-    uint32_t zchunks, tzc, nitems;
+    uint32_t tzc;        // planes per z-chunk
+    uint32_t zc0, nzc;   // this part's z-chunks [zc0, zc0 + nzc)
+    uint32_t zlo, zhi;   // this part's planes [zlo, zhi)
+    uint32_t nitems;     // W * Y * nzc
 };
 
-__device__ __forceinline__ JobGeo job_geo(const DevJob &j, uint32_t zchunks)
+__device__ __forceinline__ JobGeo job_geo(const DevJob &j)
 {
     JobGeo g;
     g.X = j.X;
@@ -48,9 +51,16 @@ __device__ __forceinline__ JobGeo job_geo(const DevJob &j, uint32_t zchunks)
     g.aplane = g.aslab * (j.Z + 2u);
     const uint32_t r = j.X - 32u * (g.W - 1u);
     g.last_mask = r >= 32u ? ~0u : ((1u << r) - 1u);
+    const uint32_t zchunks = max(1u, j.zchunks);
+    const uint32_t parts = max(1u, j.parts);
     g.tzc = (j.Z + zchunks - 1u) / zchunks;
-    g.zchunks = (j.Z + g.tzc - 1u) / g.tzc;
-    g.nitems = g.W * j.Y * g.zchunks;
+    const uint32_t used = (j.Z + g.tzc - 1u) / g.tzc;  // non-empty chunks
+    const uint32_t c = zchunks / parts;
+    g.zc0 = min(j.part * c, used);
+    g.nzc = min(g.zc0 + c, used) - g.zc0;
+    g.zlo = min(g.zc0 * g.tzc, j.Z);
+    g.zhi = min((g.zc0 + g.nzc) * g.tzc, j.Z);
+    g.nitems = g.W * j.Y * g.nzc;
     return g;
 }
 
@@ -60,10 +70,37 @@ struct BCarry {
     uint32_t A;   // own live word of the last input plane
 };
 
+// Live-plane words that another block reads or writes (the first and last
+// plane of a part's slab, and the planes just outside it) go through
+// device-scope atomic loads / stores: on gfx950 these are `sc1` accesses that
+// are coherent across the XCDs' L2 caches without a cache-wide write-back or
+// invalidate, so a split grid does not disturb the other blocks' cached cells.
+template <bool COH>
+__device__ __forceinline__ uint32_t ld_live(const uint32_t *p, bool shared)
+{
+    if constexpr (COH) {
+        if (shared) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return *p;
+}
+
+template <bool COH>
+__device__ __forceinline__ void st_live(uint32_t *p, uint32_t v, bool shared)
+{
+    if constexpr (COH) {
+        if (shared) {
+            __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+    *p = v;
+}
+
 // One item: column (w, y), planes [z0, z1).  A_cur / A_next / P are offsets
 // into S.  Loads of the x-neighbour words of a row always hit valid storage
 // (the previous / next row or a guard row); words outside the row are masked.
-template <uint32_t FLAGS>
+// COH: the job is split over several blocks (boundary planes are shared).
+template <uint32_t FLAGS, bool COH>
 __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext,
                                          const RuleConsts &rc, int w, int y, int z0, int z1, StatAcc &acc)
 {
@@ -78,14 +115,30 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
         RowSums rs[3];
         const uint32_t *a0 = A + g.aslab * (uint32_t)(zi + 1);
+        // planes zlo-1, zlo, zhi-1, zhi are written or read by a neighbouring part
+        const bool shr = COH && (zi <= (int)g.zlo || zi + 1 >= (int)g.zhi);
+        uint32_t av[3], pv[3], nv[3];
+        if (shr) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const uint32_t *p = a0 + d * g.W;
-            const uint32_t a = p[0];
-            const uint32_t prev = p[-1] & pm;
-            const uint32_t next = p[1] & nm;
-            rs[d] = row_sums(a, shift_left_neighbor_nat(a, prev), shift_right_neighbor_nat(a, next));
+            for (int d = 0; d < 3; d++) {
+                const uint32_t *p = a0 + d * g.W;
+                av[d] = ld_live<COH>(p, true);
+                pv[d] = ld_live<COH>(p - 1, true);
+                nv[d] = ld_live<COH>(p + 1, true);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const uint32_t *p = a0 + d * g.W;
+                av[d] = p[0];
+                pv[d] = p[-1];
+                nv[d] = p[1];
+            }
         }
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            rs[d] = row_sums(av[d], shift_left_neighbor_nat(av[d], pv[d] & pm),
+                             shift_right_neighbor_nat(av[d], nv[d] & nm));
         const Bits<4> S0c = add3<4>(kind1<FLAGS>(rs[0]), kind0<FLAGS>(rs[1]), kind1<FLAGS>(rs[2]));
         const Bits<4> S1c = add3<4>(kind2<FLAGS>(rs[0]), kind1<FLAGS>(rs[1]), kind2<FLAGS>(rs[2]));
         co.Q = add3<5>(ci.S1p, S0c, Bits<1>{{0u}});
@@ -110,7 +163,8 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
             P[g.pplane] = pl[1];
             P[2u * g.pplane] = pl[2];
             P[3u * g.pplane] = pl[3];
-            S[anext + (uint32_t)w + g.W * ((uint32_t)(y + 1) + (g.Y + 2u) * (uint32_t)(z + 1))] = set_m;
+            st_live<COH>(S + anext + (uint32_t)w + g.W * ((uint32_t)(y + 1) + (g.Y + 2u) * (uint32_t)(z + 1)), set_m,
+                         COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
             colmask |= set_m;
             if (set_m) {
                 zmin = min(zmin, z);
@@ -139,7 +193,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
     }
 }
 
-template <uint32_t FLAGS>
+template <uint32_t FLAGS, bool COH>
 __device__ void tick_items(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext, const RuleConsts &rc,
                            StatAcc &acc)
 {
@@ -147,36 +201,50 @@ __device__ void tick_items(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t
         const int w = (int)(it % g.W);
         const uint32_t rest = it / g.W;
         const int y = (int)(rest % g.Y);
-        const uint32_t zc = rest / g.Y;
+        const uint32_t zc = g.zc0 + rest / g.Y;
         const int z0 = (int)(zc * g.tzc);
         const int z1 = min(z0 + (int)g.tzc, (int)g.Z);
-        run_item<FLAGS>(g, S, acur, anext, rc, w, y, z0, z1, acc);
+        run_item<FLAGS, COH>(g, S, acur, anext, rc, w, y, z0, z1, acc);
     }
 }
 
+template <bool COH>
 __device__ void run_tick(uint32_t flags, const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext,
                          const RuleConsts &rc, StatAcc &acc)
 {
     switch (flags & 7u) {
-    case 1: tick_items<1>(g, S, acur, anext, rc, acc); break;
-    case 2: tick_items<2>(g, S, acur, anext, rc, acc); break;
-    case 3: tick_items<3>(g, S, acur, anext, rc, acc); break;
-    case 4: tick_items<4>(g, S, acur, anext, rc, acc); break;
-    case 5: tick_items<5>(g, S, acur, anext, rc, acc); break;
-    case 6: tick_items<6>(g, S, acur, anext, rc, acc); break;
-    case 7: tick_items<7>(g, S, acur, anext, rc, acc); break;
-    default: tick_items<0>(g, S, acur, anext, rc, acc); break;
+    case 1: tick_items<1, COH>(g, S, acur, anext, rc, acc); break;
+    case 2: tick_items<2, COH>(g, S, acur, anext, rc, acc); break;
+    case 3: tick_items<3, COH>(g, S, acur, anext, rc, acc); break;
+    case 4: tick_items<4, COH>(g, S, acur, anext, rc, acc); break;
+    case 5: tick_items<5, COH>(g, S, acur, anext, rc, acc); break;
+    case 6: tick_items<6, COH>(g, S, acur, anext, rc, acc); break;
+    case 7: tick_items<7, COH>(g, S, acur, anext, rc, acc); break;
+    default: tick_items<0, COH>(g, S, acur, anext, rc, acc); break;
     }
 }
 
-// Initial cells from the hash (searchSystem.cpp:264-303), natural bit order;
-// every live-plane word (guards included) is cleared first.
+// Initial cells from the hash (searchSystem.cpp:264-303), natural bit order,
+// for this part's planes [zlo, zhi).  A part owns the live-plane slabs of its
+// planes (the first part also the z = -1 guard slab, the last part the z = Z
+// one): it clears them in both live planes, then writes its live bits.
+template <bool COH>
 __device__ void init_cells(const DevJob &j, const JobGeo &g, uint32_t *S, uint32_t acur)
 {
-    for (uint32_t i = threadIdx.x; i < 2u * g.aplane; i += blockDim.x) S[4u * g.pplane + i] = 0u;
+    const uint32_t parts = max(1u, j.parts);
+    const uint32_t s0 = j.part == 0u ? 0u : g.zlo + 1u;
+    const uint32_t s1 = j.part + 1u >= parts ? g.Z + 2u : g.zhi + 1u;
+    const uint32_t nclr = (s1 > s0 ? s1 - s0 : 0u) * g.aslab;
+    for (uint32_t i = threadIdx.x; i < nclr; i += blockDim.x) {
+        st_live<COH>(S + 4u * g.pplane + s0 * g.aslab + i, 0u, true);
+        st_live<COH>(S + 4u * g.pplane + g.aplane + s0 * g.aslab + i, 0u, true);
+    }
     __syncthreads();
     const uint32_t live = j.M <= 15u ? ~0u : 0u;
-    for (uint32_t idx = threadIdx.x; idx < g.pplane; idx += blockDim.x) {
+    const uint32_t base = g.W * g.Y * g.zlo;
+    const uint32_t n = g.W * g.Y * (g.zhi - g.zlo);
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const uint32_t idx = base + k;
         const uint32_t w = idx % g.W;
         const uint32_t rest = idx / g.W;
         const uint32_t y = rest % g.Y;
@@ -192,16 +260,17 @@ __device__ void init_cells(const DevJob &j, const JobGeo &g, uint32_t *S, uint32
         S[g.pplane + idx] = (j.M & 2u) ? bits : 0u;
         S[2u * g.pplane + idx] = (j.M & 4u) ? bits : 0u;
         S[3u * g.pplane + idx] = (j.M & 8u) ? bits : 0u;
-        S[acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u))] = bits & live;
+        st_live<COH>(S + acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u)), bits & live, true);
     }
 }
 
-// Final cells in the reference packed layout (ceil(X/8) words per row).
+// Final cells of this part's planes in the reference packed layout
+// (ceil(X/8) words per row).
 __device__ void store_final(const JobGeo &g, const uint32_t *S, uint32_t *out)
 {
     const uint32_t upr = (g.X + 7u) / 8u;
-    const uint32_t n = upr * g.Y * g.Z;
-    for (uint32_t rw = threadIdx.x; rw < n; rw += blockDim.x) {
+    const uint32_t n = upr * g.Y * g.zhi;
+    for (uint32_t rw = upr * g.Y * g.zlo + threadIdx.x; rw < n; rw += blockDim.x) {
         const uint32_t xw = rw % upr;
         const uint32_t row = rw / upr;
         const uint32_t idx = (xw >> 2) + g.W * row;
@@ -265,6 +334,71 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch,
     }
 }
 
+// Rendezvous of the parts of a multi-part job at the end of a phase (the
+// init, then each tick).  Every wave first waits until its device-scope
+// stores (the shared boundary planes) have completed; thread 0 then adds the
+// block's partials to the phase's counter slot, arrives, and waits until
+// every part has arrived, then reads the job totals.  Everything another
+// block reads is written and read with device-scope atomics, and each wave's
+// stores complete (vmcnt 0) before the arrival is issued, so a part that has
+// seen every arrival reads the others' planes of this phase.  A wait is
+// bounded: past `wait_limit` the job is aborted for all parts.  Returns false
+// (to every thread) when the job was aborted.
+__device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t phase, const uint32_t part_sum[3],
+                                const int32_t bmin[3], const int32_t bmax[3], uint32_t tot[3], int32_t tmin[3],
+                                int32_t tmax[3], uint32_t *s_flag)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        JobSync *js = a.sync + j.sync_slot;
+        const uint32_t slot = phase % 3u;
+        if (phase > 0) {
+            for (int k = 0; k < 3; k++) {
+                __hip_atomic_fetch_add(&js->sum[slot][k], part_sum[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_max(&js->nmin[slot][k], 0x7fffffff - bmin[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_max(&js->xmax1[slot][k], bmax[k] + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        // the slot of phase + 1 was last read in phase - 2, before every part
+        // arrived for phase - 1: the first part clears it for reuse
+        if (j.part == 0u && phase >= 2u) {
+            const uint32_t ns = (phase + 1u) % 3u;
+            for (int k = 0; k < 3; k++) {
+                __hip_atomic_store(&js->sum[ns][k], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&js->nmin[ns][k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&js->xmax1[ns][k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_fetch_add(&js->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t target = j.parts * (phase + 1u);
+        const uint64_t t0 = wall_clock64();
+        uint32_t ok = 1u;
+        while (__hip_atomic_load(&js->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&js->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > a.wait_limit) {
+                __hip_atomic_store(&js->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0u;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (ok && phase > 0) {
+            for (int k = 0; k < 3; k++) {
+                tot[k] = __hip_atomic_load(&js->sum[slot][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tmin[k] = 0x7fffffff -
+                          __hip_atomic_load(&js->nmin[slot][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tmax[k] = __hip_atomic_load(&js->xmax1[slot][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
+            }
+        }
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0u;
+}
+
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
     extern __shared__ uint32_t lds[];
@@ -272,6 +406,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
     __shared__ int32_t s_box[6 * 16];
     __shared__ uint32_t s_job;
     __shared__ uint32_t s_done;
+    __shared__ uint32_t s_ok;
 
     uint32_t *arena = a.arena ? a.arena + (size_t)blockIdx.x * a.arena_words : nullptr;
     while (true) {
@@ -281,22 +416,40 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];
-        // z-chunks per column chosen on the host (one item per thread when possible)
-        const JobGeo g = job_geo(j, max(1u, j.zchunks));
+        const uint64_t t_start = wall_clock64();
+        const JobGeo g = job_geo(j);
         const RuleConsts rc = rule_consts(j.M, j.born_lut, j.surv_lut);
+        const bool multi = j.parts > 1u;
+        uint32_t *S = j.tier == kTierOwnRegion ? a.job_arena + j.region : arena;
         uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;
-        if (j.tier == 0) init_cells(j, g, lds, acur);
-        else init_cells(j, g, arena, acur);
+        if (j.tier == kTierLds) init_cells<false>(j, g, lds, acur);
+        else if (multi) init_cells<true>(j, g, S, acur);
+        else init_cells<false>(j, g, S, acur);
         CaveGridInfo info{};
         CaveTickStats last{};
         const uint32_t cells = j.X * j.Y * j.Z;
-        __syncthreads();
-        for (uint32_t t = 0; t < j.ticks; t++) {
+        bool ok = true;
+        if (multi) {
+            uint32_t t3[3] = {0u, 0u, 0u};
+            int32_t b3[3] = {0, 0, 0}, c3[3] = {0, 0, 0};
+            ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &s_ok);
+        } else {
+            __syncthreads();
+        }
+        for (uint32_t t = 0; ok && t < j.ticks; t++) {
             StatAcc acc;
-            if (j.tier == 0) run_tick(j.flags, g, lds, acur, anext, rc, acc);
-            else run_tick(j.flags, g, arena, acur, anext, rc, acc);
+            if (j.tier == kTierLds) run_tick<false>(j.flags, g, lds, acur, anext, rc, acc);
+            else if (multi) run_tick<true>(j.flags, g, S, acur, anext, rc, acc);
+            else run_tick<false>(j.flags, g, S, acur, anext, rc, acc);
             uint32_t tot[3];
+            int32_t bmin[3], bmax[3];
             block_sum3(acc.dying, acc.born, acc.surv, s_sum, tot);
+            if (multi) {
+                // every tick: whether this tick is the last is known only after the totals
+                block_bbox(acc, s_box, bmin, bmax);
+                ok = part_rendezvous(a, j, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &s_ok);
+                if (!ok) break;
+            }
             if (threadIdx.x == 0) {
                 const uint32_t dying = tot[0], born = tot[1], surv = tot[2];
                 const uint32_t dead = cells - dying - born - surv;
@@ -321,16 +474,13 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
             }
             __syncthreads();
             // the bounding box is reduced only when someone reads it
-            if (s_done || a.per_tick) {
-                int32_t bmin[3], bmax[3];
-                block_bbox(acc, s_box, bmin, bmax);
-                if (threadIdx.x == 0) {
-                    for (int k = 0; k < 3; k++) {
-                        last.bbox_min[k] = bmin[k];
-                        last.bbox_max[k] = bmax[k];
-                    }
-                    if (a.per_tick) a.per_tick[(size_t)j.result_slot * a.per_tick_stride + t] = last;
+            if (!multi && (s_done || a.per_tick)) block_bbox(acc, s_box, bmin, bmax);
+            if (threadIdx.x == 0 && (s_done || a.per_tick)) {
+                for (int k = 0; k < 3; k++) {
+                    last.bbox_min[k] = bmin[k];
+                    last.bbox_max[k] = bmax[k];
                 }
+                if (a.per_tick && j.part == 0u) a.per_tick[(size_t)j.result_slot * a.per_tick_stride + t] = last;
             }
             const uint32_t tmp = acur;
             acur = anext;
@@ -339,23 +489,29 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
             __syncthreads();
             if (done) break;
         }
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && j.part == 0u) {
             DevResult r;
             r.info = info;
             // searchSystem.cpp:695-711
             r.classification = info.status == 2u ? 2u : (info.numberOfSimulationTicksSurvived > j.M + 1u ? 1u : 0u);
             r.ticks_run = info.numberOfSimulationTicksSurvived;
             r.last = last;
+            r.t_start = t_start;
+            r.t_end = wall_clock64();
+            r.err = ok ? 0u : 1u;
+            r.pad_ = 0u;
             a.results[j.result_slot] = r;
         }
-        if (j.final_slot) {
+        if (j.final_slot && ok) {
             uint32_t *dst = a.final_cells + a.final_offsets[j.final_slot - 1u];
-            if (j.tier == 0) store_final(g, lds, dst);
-            else store_final(g, arena, dst);
+            if (j.tier == kTierLds) store_final(g, lds, dst);
+            else store_final(g, S, dst);
         }
         __syncthreads();
     }
 }
+
+const void *batch_kernel() { return (const void *)k_batch; }
 
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st)
 {

@@ -1,0 +1,494 @@
+// batch_plan.cpp -- layout of a simulation list on the persistent batched
+// kernel, the autotuner behind it, and the launch itself.
+//
+// Autotuner.  Throughput of the batched kernel depends on the grid size and
+// neighbourhood of the simulations (bucketed by grid edge and F/E/C flags), on
+// the block size (which also decides which grids fit the LDS: 1024-thread
+// blocks get a whole CU's LDS, 512 half of it, 256 a quarter), and on how
+// finely a tick is cut into work items (items per thread).  For each bucket
+// present in a batch and each candidate (block size, items per thread) the
+// tuner times one short calibration launch in which every resident block runs
+// a grid of that bucket, and keeps the device-wide rate (cell-updates/s) in a
+// process-wide table.  The planner then picks the block size with the lowest
+// predicted time for the batch's bucket mix, the items per thread per bucket,
+// predicts every simulation's single-block time from its bucket's rate, splits
+// the simulations that would outlast the batch's average block load over
+// several blocks, and orders the queue (split simulations first, so that all
+// of their parts start together, then longest first).
+#include "batch_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "host_util.h"
+
+namespace cave {
+
+uint64_t fnv_ref_layout(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z);
+
+namespace {
+
+constexpr uint32_t kLdsBytesPerCU = 160u * 1024u;
+constexpr uint32_t kStaticLds = 1024u;    // s_sum / s_box / scalars of k_batch
+constexpr uint64_t kPartWaitSeconds = 4;  // a split simulation's parts give up waiting after this
+constexpr uint32_t kThreadChoices[] = {256u, 512u, 1024u};
+constexpr uint32_t kIptChoices[] = {1u, 2u};
+constexpr uint32_t kDefaultMaxParts = 16u;
+constexpr double kTuneMargin = 1.03;  // a tuned choice must beat the default by 3%
+
+struct TuneKey {
+    uint32_t bucket, flags, threads, ipt;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(bucket, flags, threads, ipt) < std::tie(o.bucket, o.flags, o.threads, o.ipt);
+    }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, double> g_rates;  // device-wide cell-updates/s, every resident block on the bucket
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+int dev_alloc(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? CAVE_E_NOMEM : CAVE_E_DEVICE,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return CAVE_OK;
+}
+
+uint64_t wall_clock_khz()
+{
+    static int khz = 0;
+    if (!khz && (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0))
+        khz = 100000;
+    return (uint64_t)khz;
+}
+
+// LDS a block may use so that 1024 / threads blocks share one CU.
+uint32_t lds_cap(uint32_t threads)
+{
+    const uint32_t per = std::max(1u, 1024u / threads);
+    return kLdsBytesPerCU / per - kStaticLds;
+}
+
+double split_divisor()
+{
+    const char *e = std::getenv("CAVE_SPLIT_DIV");
+    const double d = e ? std::atof(e) : 8.0;
+    return d > 0.0 ? d : 8.0;
+}
+
+bool tuner_enabled()
+{
+    const char *e = std::getenv("CAVE_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+
+// One calibration launch: every resident block runs a copy of a grid of the
+// bucket (all-counts rule, no early exit, a few ticks).
+int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double &rate)
+{
+    const uint32_t copies = cu_count() * std::max(1u, 1024u / key.threads);
+    const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
+    const uint32_t N = neighbor_count(key.flags);
+    const uint64_t all = N ? (N >= 64 ? ~0ull : ((1ull << N) - 1)) : 1ull;
+    const double target = 2e9;
+    const uint32_t ticks = (uint32_t)std::clamp<double>(target / ((double)copies * (double)cells), 2.0, 64.0);
+    std::vector<CaveSimSpec> specs(copies);
+    for (uint32_t i = 0; i < copies; i++) {
+        CaveSimSpec &s = specs[i];
+        std::memset(&s, 0, sizeof s);
+        for (int a = 0; a < 3; a++) {
+            s.dims[a] = dims[a];
+            s.spawn[a] = std::max(1u, dims[a] / 2u);
+        }
+        s.max_cell_state = 7;
+        s.neighborhood_flags = key.flags;
+        s.ticks = ticks;
+        s.birth = all;
+        s.survival = all;
+        s.spawn_mode = CAVE_SPAWN_RANDOM;
+        s.density = 0.25;
+        s.seed = i;
+        s.flags = CAVE_SIM_NO_EARLY_EXIT;
+    }
+    BatchKnobs k;
+    k.threads = key.threads;
+    k.ipt = key.ipt;
+    k.max_parts = 1;
+    k.tune = false;
+    BatchPlan plan;
+    int rc = plan_batch(specs.data(), copies, k, plan);
+    if (rc) return rc;
+    std::vector<CaveSimResult> res(copies);
+    CaveBatchOptions opt{};
+    if ((rc = run_plan(plan, copies, res.data(), &opt, st))) return rc;
+    rate = (double)copies * (double)cells * ticks / (std::max(opt.device_ms, 1e-3) * 1e-3);
+    return CAVE_OK;
+}
+
+struct BucketWork {
+    uint32_t dims[3];  // a representative grid of the bucket
+    double work = 0;   // budgeted cell-updates
+};
+
+}  // namespace
+
+uint32_t cu_count()
+{
+    static int cus = 0;
+    if (!cus) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 256;
+        cus = p.multiProcessorCount;
+    }
+    return (uint32_t)cus;
+}
+
+uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes)
+{
+    int per = 0;
+    if (lds_bytes > 64u * 1024u)
+        (void)hipFuncSetAttribute(batch_kernel(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(), (int)threads, lds_bytes) != hipSuccess ||
+        per < 1)
+        per = 1;
+    return (uint32_t)per;
+}
+
+uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z)
+{
+    const double e = std::cbrt((double)X * Y * Z);
+    if (e < 32) return 0;
+    if (e < 48) return 1;
+    if (e < 64) return 2;
+    if (e < 96) return 3;
+    if (e < 128) return 4;
+    return 5;
+}
+
+DevJob make_job(const CaveSimSpec &s, uint32_t slot)
+{
+    DevJob j{};
+    j.X = s.dims[0];
+    j.Y = s.dims[1];
+    j.Z = s.dims[2];
+    j.flags = s.neighborhood_flags & 7u;
+    j.M = s.max_cell_state;
+    const DevRule r = make_rule(s.max_cell_state, s.neighborhood_flags, s.birth, s.survival);
+    j.born_lut = r.born_lut;
+    j.surv_lut = r.surv_lut;
+    j.ticks = s.ticks;
+    j.sim_flags = s.flags;
+    j.result_slot = slot;
+    j.parts = 1;
+    j.init = make_init(s.dims, s.spawn, s.spawn_mode, s.density, s.seed);
+    return j;
+}
+
+int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPlan &plan)
+{
+    plan = BatchPlan{};
+    plan.jobs.resize(n);
+    plan.ipt.resize(n);
+    std::vector<uint64_t> words(n);
+    std::vector<double> cost(n);
+    std::vector<uint32_t> key_bucket(n);
+    std::map<std::pair<uint32_t, uint32_t>, BucketWork> buckets;  // (edge bucket, flags)
+    for (size_t i = 0; i < n; i++) {
+        DevJob &j = plan.jobs[i];
+        j = make_job(specs[i], (uint32_t)i);
+        words[i] = job_words(j.X, j.Y, j.Z);
+        cost[i] = (double)j.X * j.Y * j.Z * j.ticks;
+        key_bucket[i] = edge_bucket(j.X, j.Y, j.Z);
+        BucketWork &b = buckets[{key_bucket[i], j.flags}];
+        if (b.work == 0)
+            for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
+        b.work += cost[i];
+        if (specs[i].flags & CAVE_SIM_FINAL_HASH) {
+            plan.final_offsets.push_back(plan.final_words);
+            j.final_slot = (uint32_t)plan.final_offsets.size();
+            plan.final_words += cave_cell_words(j.X, j.Y, j.Z);
+        }
+    }
+
+    // ---- block size and items per thread from the rate table ----
+    const bool tune = k.tune && tuner_enabled();
+    std::vector<uint32_t> cand;
+    if (k.threads) cand.push_back(std::max(64u, std::min(1024u, (k.threads + 63u) / 64u * 64u)));
+    else if (tune) cand.assign(std::begin(kThreadChoices), std::end(kThreadChoices));
+    else cand.push_back(1024u);
+    std::vector<uint32_t> ipts;  // the first entry is the default
+    if (k.ipt) ipts.push_back(k.ipt);
+    else if (tune) ipts.assign(std::begin(kIptChoices), std::end(kIptChoices));
+    else ipts.push_back(1u);
+
+    auto rate_of = [&](uint32_t bucket, uint32_t flags, uint32_t T, uint32_t ipt) -> double {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_rates.find(TuneKey{bucket, flags, T, ipt});
+        return it == g_rates.end() ? 0.0 : it->second;
+    };
+    if (tune) {
+        hipStream_t cst;
+        CAVE_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
+        int rc = CAVE_OK;
+        for (const auto &[bk, bw] : buckets)
+            for (uint32_t T : cand)
+                for (uint32_t ipt : ipts) {
+                    if (rc) break;
+                    const TuneKey key{bk.first, bk.second, T, ipt};
+                    if (rate_of(key.bucket, key.flags, T, ipt) > 0) continue;
+                    double r = 0;
+                    rc = calibrate(key, bw.dims, cst, r);
+                    if (!rc) {
+                        std::lock_guard<std::mutex> g(g_tune_mu);
+                        g_rates[key] = r;
+                    }
+                }
+        (void)hipStreamSynchronize(cst);
+        (void)hipStreamDestroy(cst);
+        if (rc) return rc;
+    }
+    // a candidate replaces the default (the first one listed) only when it
+    // measured kTuneMargin faster: calibration noise does not flip choices
+    auto best_ipt = [&](uint32_t bucket, uint32_t flags, uint32_t T, double *rate) {
+        uint32_t bi = ipts[0];
+        double br = rate_of(bucket, flags, T, ipts[0]);
+        for (uint32_t ipt : ipts) {
+            const double r = rate_of(bucket, flags, T, ipt);
+            if (r > br * kTuneMargin) {
+                br = r;
+                bi = ipt;
+            }
+        }
+        if (rate) *rate = br;
+        return bi;
+    };
+    uint32_t threads = cand.back();  // the largest block size is the default
+    if (tune && cand.size() > 1) {
+        auto predict = [&](uint32_t T) {
+            double t = 0;
+            for (const auto &[bk, bw] : buckets) {
+                double r = 0;
+                best_ipt(bk.first, bk.second, T, &r);
+                t += bw.work / std::max(r, 1.0);
+            }
+            return t;
+        };
+        double best = predict(threads);
+        for (uint32_t T : cand) {
+            const double t = predict(T);
+            if (t * kTuneMargin < best) {
+                best = t;
+                threads = T;
+            }
+        }
+    }
+    plan.threads = threads;
+
+    // ---- storage tiers, residency ----
+    const uint32_t cap = lds_cap(threads);
+    uint64_t lds_words = 0;
+    for (size_t i = 0; i < n; i++) {
+        DevJob &j = plan.jobs[i];
+        if (words[i] * 4u <= cap) {
+            j.tier = kTierLds;
+            lds_words = std::max(lds_words, words[i]);
+        } else {
+            j.tier = kTierBlockArena;
+        }
+    }
+    plan.lds_bytes = (uint32_t)(lds_words * 4u);
+    const uint32_t per_cu = resident_per_cu(threads, plan.lds_bytes);
+    const uint32_t resident = cu_count() * per_cu;
+
+    // ---- single-block time estimates, splitting, order ----
+    std::vector<double> t(n);
+    std::vector<uint32_t> ipt_of(n, ipts[0]);
+    double total = 0;
+    for (size_t i = 0; i < n; i++) {
+        const DevJob &j = plan.jobs[i];
+        double r = 0;
+        ipt_of[i] = best_ipt(key_bucket[i], j.flags, threads, &r);
+        // a bucket's device-wide rate is shared by `resident` blocks
+        t[i] = r > 0 ? cost[i] * resident / r : cost[i];
+        total += t[i];
+    }
+    const double share = total / resident / split_divisor();
+    uint32_t max_parts = k.max_parts ? k.max_parts : kDefaultMaxParts;
+    max_parts = std::min(max_parts, std::max(1u, resident / 4u));
+    std::vector<std::tuple<int, double, size_t>> order(n);  // (unsplit, per-part time, job)
+    double longest_part = 0;
+    for (size_t i = 0; i < n; i++) {
+        DevJob &j = plan.jobs[i];
+        uint32_t parts = 1;
+        if (j.tier != kTierLds && max_parts > 1 && t[i] > share) {
+            const double want = std::ceil(t[i] / share);
+            parts = (uint32_t)std::min<double>(want, (double)std::min(max_parts, std::max(1u, j.Z / 4u)));
+            parts = std::max(1u, parts);
+        }
+        j.parts = parts;
+        j.zchunks = part_zchunks(j.X, j.Y, j.Z, threads * ipt_of[i], parts);
+        plan.ipt[i] = ipt_of[i];
+        if (parts > 1) {
+            j.tier = kTierOwnRegion;
+            j.region = plan.region_words;
+            plan.region_words += (words[i] + 63u) / 64u * 64u;
+            j.sync_slot = plan.nsync++;
+        } else if (j.tier == kTierBlockArena) {
+            plan.arena_words = std::max(plan.arena_words, words[i]);
+        }
+        order[i] = {parts > 1 ? 0 : 1, t[i] / parts, i};
+        longest_part = std::max(longest_part, t[i] / parts);
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto &x, const auto &y) {
+        if (std::get<0>(x) != std::get<0>(y)) return std::get<0>(x) < std::get<0>(y);
+        return std::get<1>(x) > std::get<1>(y);
+    });
+    plan.queue.reserve(n);
+    for (const auto &o : order) {
+        const DevJob &j = plan.jobs[std::get<2>(o)];
+        for (uint32_t p = 0; p < j.parts; p++) {
+            DevJob e = j;
+            e.part = p;
+            plan.queue.push_back(e);
+        }
+    }
+    plan.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(resident, plan.queue.size()));
+    if (tune) plan.predicted_ms = std::max(total / resident, longest_part) * 1e3;
+    return CAVE_OK;
+}
+
+int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
+{
+    const size_t nq = plan.queue.size();
+    DevBuf djobs, dnext, dres, dpt, darena, dfinal, dfoff, dregion, dsync;
+    int rc;
+    if ((rc = dev_alloc(djobs, sizeof(DevJob) * nq))) return rc;
+    if ((rc = dev_alloc(dnext, sizeof(uint32_t)))) return rc;
+    if ((rc = dev_alloc(dres, sizeof(DevResult) * n))) return rc;
+    const uint32_t stride = opt ? opt->per_tick_stride : 0u;
+    const bool want_pt = opt && opt->per_tick && stride;
+    if (want_pt && (rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
+    if (plan.arena_words && (rc = dev_alloc(darena, sizeof(uint32_t) * plan.arena_words * plan.blocks))) return rc;
+    if (plan.region_words && (rc = dev_alloc(dregion, sizeof(uint32_t) * plan.region_words))) return rc;
+    if (plan.nsync) {
+        if ((rc = dev_alloc(dsync, sizeof(JobSync) * plan.nsync))) return rc;
+        CAVE_HIP(hipMemsetAsync(dsync.p, 0, sizeof(JobSync) * plan.nsync, st));
+    }
+    if (plan.final_words) {
+        if ((rc = dev_alloc(dfinal, sizeof(uint32_t) * plan.final_words))) return rc;
+        if ((rc = dev_alloc(dfoff, sizeof(uint64_t) * plan.final_offsets.size()))) return rc;
+        CAVE_HIP(hipMemcpyAsync(dfoff.p, plan.final_offsets.data(), sizeof(uint64_t) * plan.final_offsets.size(),
+                                hipMemcpyHostToDevice, st));
+    }
+    CAVE_HIP(hipMemcpyAsync(djobs.p, plan.queue.data(), sizeof(DevJob) * nq, hipMemcpyHostToDevice, st));
+    CAVE_HIP(hipMemsetAsync(dnext.p, 0, sizeof(uint32_t), st));
+
+    BatchArgs a{};
+    a.jobs = djobs.as<DevJob>();
+    a.njobs = (uint32_t)nq;
+    a.next_job = dnext.as<uint32_t>();
+    a.results = dres.as<DevResult>();
+    a.per_tick = want_pt ? dpt.as<CaveTickStats>() : nullptr;
+    a.per_tick_stride = stride;
+    a.arena = darena.as<uint32_t>();
+    a.arena_words = plan.arena_words;
+    a.final_cells = dfinal.as<uint32_t>();
+    a.final_offsets = dfoff.as<uint64_t>();
+    a.lds_words = plan.lds_bytes / 4u;
+    a.job_arena = dregion.as<uint32_t>();
+    a.sync = dsync.as<JobSync>();
+    a.wait_limit = wall_clock_khz() * 1000u * kPartWaitSeconds;
+
+    hipEvent_t e0, e1;
+    CAVE_HIP(hipEventCreate(&e0));
+    CAVE_HIP(hipEventCreate(&e1));
+    CAVE_HIP(hipEventRecord(e0, st));
+    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st);
+    if (rc) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+    CAVE_HIP(hipEventRecord(e1, st));
+    std::vector<DevResult> hres(n);
+    CAVE_HIP(hipMemcpyAsync(hres.data(), dres.p, sizeof(DevResult) * n, hipMemcpyDeviceToHost, st));
+    if (want_pt)
+        CAVE_HIP(hipMemcpyAsync(opt->per_tick, dpt.p, sizeof(CaveTickStats) * n * stride, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> hfinal(plan.final_words);
+    if (plan.final_words)
+        CAVE_HIP(hipMemcpyAsync(hfinal.data(), dfinal.p, sizeof(uint32_t) * plan.final_words, hipMemcpyDeviceToHost,
+                                st));
+    CAVE_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    CAVE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (opt) opt->device_ms = ms;
+
+    for (size_t i = 0; i < n; i++)
+        if (hres[i].err)
+            return fail(CAVE_E_STATE, "the parts of a split simulation did not meet within the wait limit");
+    for (size_t i = 0; i < n; i++) {
+        CaveSimResult &r = results[i];
+        std::memset(&r, 0, sizeof r);
+        r.info = hres[i].info;
+        r.classification = hres[i].classification;
+        r.ticks_run = hres[i].ticks_run;
+        r.last = hres[i].last;
+    }
+    if (opt && opt->timing) {
+        uint64_t t0 = ~0ull;
+        for (size_t i = 0; i < n; i++) t0 = std::min(t0, hres[i].t_start);
+        const double us_per_tick = 1000.0 / (double)wall_clock_khz();
+        for (size_t i = 0; i < n; i++) {
+            CaveSimTiming &tm = opt->timing[i];
+            tm.start_us = (double)(hres[i].t_start - t0) * us_per_tick;
+            tm.end_us = (double)(hres[i].t_end - t0) * us_per_tick;
+            tm.blocks = plan.jobs[i].parts;
+            tm.items_per_thread = plan.ipt[i];
+        }
+    }
+    for (const DevJob &j : plan.jobs)
+        if (j.final_slot)
+            results[j.result_slot].final_hash =
+                fnv_ref_layout(hfinal.data() + plan.final_offsets[j.final_slot - 1], j.X, j.Y, j.Z);
+    return CAVE_OK;
+}
+
+std::string tuning_report()
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::string s = "[";
+    bool first = true;
+    for (const auto &[k, r] : g_rates) {
+        if (!first) s += ",";
+        first = false;
+        s += "{\"edge_bucket\":" + std::to_string(k.bucket) + ",\"flags\":" + std::to_string(k.flags) +
+             ",\"threads\":" + std::to_string(k.threads) + ",\"items_per_thread\":" + std::to_string(k.ipt) +
+             ",\"cell_updates_per_s\":" + std::to_string(r) + "}";
+    }
+    return s + "]";
+}
+
+}  // namespace cave

@@ -1,0 +1,51 @@
+// batch_plan.h -- how a list of simulations is laid out on the persistent
+// batched kernel: block size, storage tier per simulation, z-chunk tiling,
+// splitting of long simulations over several blocks, and queue order.  The
+// choices come from a rate table that the autotuner fills by timing short
+// calibration launches per (grid-edge bucket, neighbourhood) bucket.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "batch.h"
+
+namespace cave {
+
+struct BatchKnobs {
+    uint32_t threads = 0;    // block size (0 = chosen by the tuner, 1024 without it)
+    uint32_t max_parts = 0;  // most blocks per simulation (0 = auto, 1 = never split)
+    uint32_t ipt = 0;        // work items per thread of a tick (0 = per bucket from the tuner, else fixed)
+    bool tune = true;        // consult / fill the rate table
+};
+
+struct BatchPlan {
+    uint32_t threads = 1024, blocks = 1, lds_bytes = 0;
+    std::vector<DevJob> jobs;   // one per spec, in spec order
+    std::vector<DevJob> queue;  // one entry per part, in claim order
+    std::vector<uint32_t> ipt;  // work items per thread, per spec
+    uint64_t arena_words = 0;   // per block (kTierBlockArena)
+    uint64_t region_words = 0;  // total of kTierOwnRegion regions
+    uint32_t nsync = 0;
+    std::vector<uint64_t> final_offsets;
+    uint64_t final_words = 0;
+    double predicted_ms = 0;    // the tuner's estimate of the launch (0 without tuning)
+};
+
+// Grid-edge bucket of a simulation (edge = cube root of its cell count):
+// [0,32) [32,48) [48,64) [64,96) [96,128) [128,...).
+uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z);
+
+// Blocks of `threads` threads one CU keeps resident with `lds_bytes` of
+// dynamic LDS each.
+uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes);
+uint32_t cu_count();
+DevJob make_job(const CaveSimSpec &s, uint32_t slot);
+
+int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPlan &plan);
+
+// Runs a planned batch on `st` and fills results (and the options' per-tick
+// rows, timing and device time).
+int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st);
+
+}  // namespace cave

@@ -208,15 +208,31 @@ enum {
 
 typedef struct CaveBatchOptions {
     uint32_t engine;         /* CAVE_ENGINE_* */
-    uint32_t block_threads;  /* 0 = autotuned */
+    uint32_t block_threads;  /* 0 = autotuned; receives the block size used (persistent engine) */
     CaveTickStats *per_tick; /* nullable: n_specs * max(ticks) entries, row per spec */
     uint32_t per_tick_stride; /* entries per spec row in per_tick (>= max ticks) */
-    uint32_t reserved;
+    uint32_t max_parts;      /* most blocks one simulation may be split over (0 = auto, 1 = never split) */
     double device_ms;        /* out: device time of the evaluation */
+    struct CaveSimTiming *timing; /* nullable out: one per spec (persistent engine) */
 } CaveBatchOptions;
+
+/* Where and when one simulation ran on the device (persistent engine). */
+typedef struct CaveSimTiming {
+    double start_us; /* device clock at the claim of its first part, after the launch's first claim */
+    double end_us;   /* device clock when its result was written */
+    uint32_t blocks; /* blocks the simulation was split over */
+    uint32_t items_per_thread;
+} CaveSimTiming;
 
 int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *results,
                    CaveBatchOptions *opt);
+
+/* The persistent engine's autotuner table measured so far in this process, as
+ * a JSON array of {edge_bucket, flags, threads, items_per_thread,
+ * cell_updates_per_s}: one calibration launch per (grid-edge bucket [0,32)
+ * [32,48) [48,64) [64,96) [96,128) [128,..), neighbourhood, block size, items
+ * per thread).  No reference counterpart. */
+int cave_batch_tuning_report(char *buf, size_t cap);
 
 /* ================================================================== */
 /* Reference-compatible --mode search chunk (SearchSystem equivalent)  */

@@ -238,25 +238,31 @@ static void next_permutation(orc_grid_info *info, orc_permutation *p, uint64_t m
 static uint32_t manager_tick(orc_grid_info *info, const orc_snapshot *snap, orc_permutation *perm,
                              uint64_t max_permutation, uint64_t survival_end,
                              uint32_t max_ticks, orc_grid_info *copy_info,
-                             orc_permutation *copy_perm, int legacy_death, uint64_t total_cells);
+                             orc_permutation *copy_perm, int legacy_death, uint64_t total_cells,
+                             int null_marker);
 
 uint32_t orc_manager_tick(orc_grid_info *info, const orc_snapshot *snap, orc_permutation *perm,
                           uint64_t max_permutation, uint64_t survival_end,
                           uint32_t max_ticks, orc_grid_info *copy_info,
                           orc_permutation *copy_perm)
 {
-    return manager_tick(info, snap, perm, max_permutation, survival_end, max_ticks, copy_info, copy_perm, 0, 0);
+    return manager_tick(info, snap, perm, max_permutation, survival_end, max_ticks, copy_info, copy_perm, 0, 0, 1);
 }
 
 /* legacy_death selects the superseded death test of
  * shaders/source/search/manager.comp:115 (every cell stayed dead this tick)
- * instead of searchManager.slang:128 (no survivor and no birth). */
+ * instead of searchManager.slang:128 (no survivor and no birth).
+ * null_marker: an all-zero rule word is the exhausted-slot marker
+ * (searchManager.slang:103-109) and the manager does nothing; without it the
+ * words are evaluated as the rule they spell (M = 1 with no birth count packs
+ * to an all-zero first word). */
 static uint32_t manager_tick(orc_grid_info *info, const orc_snapshot *snap, orc_permutation *perm,
                              uint64_t max_permutation, uint64_t survival_end,
                              uint32_t max_ticks, orc_grid_info *copy_info,
-                             orc_permutation *copy_perm, int legacy_death, uint64_t total_cells)
+                             orc_permutation *copy_perm, int legacy_death, uint64_t total_cells,
+                             int null_marker)
 {   /* searchManager.slang:98-149 */
-    if (perm->survivalAndNeighborhoodRules == 0 || perm->birthAndMaxCellStateRules == 0) {
+    if (null_marker && (perm->survivalAndNeighborhoodRules == 0 || perm->birthAndMaxCellStateRules == 0)) {
         *copy_info = *info;
         *copy_perm = *perm;
         return copy_info->status;
@@ -465,7 +471,7 @@ static uint32_t run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_s
         /* max_permutation / survival_end equal to the current values: the
            manager exhausts the slot right after this rule completes. */
         uint32_t status = manager_tick(&info, &st.snap, &perm, birth, survival, max_ticks, &ci, &cp,
-                                       legacy_death, (uint64_t)X * Y * Z);
+                                       legacy_death, (uint64_t)X * Y * Z, 0);
         if (status > 0 || t >= 0x7fffffffu) break;
     }
     if (info_out) *info_out = ci;

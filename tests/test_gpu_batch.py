@@ -120,3 +120,65 @@ def test_many_jobs_work_queue():
         info, orows, final = oracle_run(sims[i])
         assert res[i].info.numberOfSimulationTicksSurvived == info.numberOfSimulationTicksSurvived
         assert res[i].final_hash == O.canonical_hash(final, *sims[i].dims)
+
+
+def test_split_simulations_match_whole():
+    # global-tier grids split over 2..16 blocks (z-slabs, a rendezvous per tick)
+    # give the same results, per-tick rows and final cells as one block per grid
+    dims = [(70, 40, 30), (100, 33, 21), (65, 64, 64), (96, 50, 37)]
+    sims = random_sims(8, 5, dims, ticks=10)
+    for s in sims[::2]:
+        s.sim_flags |= cv.SIM_NO_EARLY_EXIT
+    print("whole", flush=True)
+    whole, wrows, _ = cv.batch_run(sims, per_tick=True, max_parts=1)
+    for parts in (2, 3, 7, 16):
+        print(f"max_parts={parts}", flush=True)
+        split, srows, _ = cv.batch_run(sims, per_tick=True, max_parts=parts)
+        for i in range(len(sims)):
+            assert split[i].ticks_run == whole[i].ticks_run, f"sim {i} parts {parts}"
+            assert split[i].last.as_tuple() == whole[i].last.as_tuple(), f"sim {i} parts {parts}"
+            assert split[i].final_hash == whole[i].final_hash, f"sim {i} parts {parts}"
+            assert [t.as_tuple() for t in srows[i]] == [t.as_tuple() for t in wrows[i]], f"sim {i} parts {parts}"
+    check(sims[1::2], cv.ENGINE_PERSISTENT)
+
+
+def test_autotuned_uneven_batch():
+    # the autotuner picks block size, items per thread and splits; results
+    # equal the fixed single-block layout and the oracle
+    from cave_amd import workloads as W
+
+    specs = W.search_uneven(96, seed=9)
+    specs["ticks"] = np.minimum(specs["ticks"], 24)
+    specs["sim_flags"] = cv.SIM_FINAL_HASH
+    print("tuned", flush=True)
+    res, _, tm, threads = cv.batch_run_np(specs, timing=True)
+    assert threads in (256, 512, 1024)
+    rep = cv.tuning_report()
+    assert rep and {"edge_bucket", "flags", "threads", "items_per_thread", "cell_updates_per_s"} <= set(rep[0])
+    assert (tm["end_us"] >= tm["start_us"]).all() and (tm["blocks"] >= 1).all()
+    print("fixed", flush=True)
+    ref, _ = cv.batch_run_np(specs, block_threads=1024, max_parts=1)
+    for f in ("died", "born", "ticks_survived", "status", "max_alive", "final_alive", "classification",
+              "ticks_run", "last", "final_hash"):
+        assert np.array_equal(res[f], ref[f]), f
+    cells = specs["dims"].astype(np.int64).prod(1)
+    for i in np.argsort(cells)[:12]:
+        sp = specs[i]
+        X, Y, Z = (int(v) for v in sp["dims"])
+        init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
+                            float(sp["density"]), int(sp["seed"]))
+        t, info, _, final = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
+                                       int(sp["survival"]), int(sp["ticks"]), init)
+        assert res["ticks_run"][i] == t
+        assert res["final_alive"][i] == info.finalAliveCount
+        assert res["final_hash"][i] == O.canonical_hash(final, X, Y, Z)
+
+
+@pytest.mark.parametrize("engine", [cv.ENGINE_PERSISTENT, cv.ENGINE_PLAIN])
+def test_empty_birth_or_survival_sets(engine):
+    # empty sets are ordinary rules here (M = 1 with no birth count packs to
+    # the reference's all-zero exhausted-slot word, but a batch spec is a rule)
+    sims = [cv.Sim(dims=(20, 17, 9), M=M, flags=fl, ticks=15, birth=b, survival=s, spawn=(10, 9, 5),
+                   spawn_mode=cv.SPAWN_RANDOM, density=0.4, seed=7, sim_flags=cv.SIM_FINAL_HASH)
+            for M, fl, b, s in [(1, 1, 0, 0), (1, 1, 0, 0x3F), (3, 3, 0, 0), (2, 7, 0x3, 0), (5, 1, 0, 0x6)]]
+    check(sims, engine)

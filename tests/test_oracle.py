@@ -262,3 +262,15 @@ def test_csv_pins_unviable_count_under_legacy_death():
     assert orc["viable"] + orc["possible"] == csv["viable"] + csv["possible"]
     # the early-dying rules do not reach the walls, so 25^3 agrees with 49^3 there
     assert fx["oracle"]["25_legacy_death"]["unviable"] == csv["unviable"]
+
+
+def test_run_rule_evaluates_all_zero_first_word():
+    # M = 1 with no birth count packs to an all-zero first rule word, the
+    # reference's exhausted-slot marker; as an explicit rule it still runs:
+    # nothing is born, so with no survival count everything dies in tick 1
+    init = O.init_cells(9, 9, 9, 1, (5, 5, 5), 0, 0.5, 3)
+    t, info, rows, _ = O.run_rule(9, 9, 9, 1, 1, 0, 0, 40, init)
+    assert t == 1 and info.status == 1 and rows[0].snap.numberOfCellsBorn == 0
+    # survival of every count keeps the live cells with a live neighbour
+    t, info, _, _ = O.run_rule(9, 9, 9, 1, 1, 0, 0x3F, 40, init)
+    assert info.numberOfSimulationTicksSurvived == t and t >= 1
