@@ -115,12 +115,14 @@ class SearchProgress(C.Structure):
 
 
 class ServeOptions(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_slots", C.c_uint32), ("slice_ticks", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("split_target_us", C.c_uint32),
+                ("idle_exit_us", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServeCompletion(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("result", SimResult), ("submit_s", C.c_double),
-                ("admit_s", C.c_double), ("finish_s", C.c_double)]
+                ("admit_s", C.c_double), ("finish_s", C.c_double), ("run_ms", C.c_double),
+                ("blocks", C.c_uint32), ("status", C.c_int32)]
 
 
 _lib = None
@@ -482,6 +484,70 @@ class Search:
         if getattr(self, "_h", None):
             lib().cave_search_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- continuous-batching request server ----
+class Server:
+    """Resident persistent kernel fed with rule-evaluation requests as they
+    arrive (cave_serve_*).  submit() queues and admits; poll() returns the
+    finished ones as ServeCompletion records (copies)."""
+
+    def __init__(self, hbm_budget_bytes=0, max_inflight=0, split_target_us=0, idle_exit_us=0):
+        o = ServeOptions()
+        o.hbm_budget_bytes = hbm_budget_bytes
+        o.max_inflight = max_inflight
+        o.split_target_us = split_target_us
+        o.idle_exit_us = idle_exit_us
+        h = C.c_void_p()
+        check(lib().cave_serve_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._buf = (ServeCompletion * 4096)()
+        self._spec = SimSpec()
+
+    def submit(self, sim, request_id: int) -> None:
+        """sim: a Sim or one SIM_DTYPE record."""
+        if isinstance(sim, Sim):
+            sim.to_c(self._spec)
+            spec = self._spec
+        else:
+            rec = np.ascontiguousarray(np.asarray(sim, dtype=SIM_DTYPE).reshape(1))
+            spec = SimSpec.from_buffer_copy(rec.tobytes())
+        check(lib().cave_serve_submit(self._h, C.byref(spec), request_id))
+
+    def poll(self):
+        out = []
+        n = C.c_size_t(0)
+        while True:
+            check(lib().cave_serve_poll(self._h, self._buf, len(self._buf), C.byref(n)))
+            out.extend(ServeCompletion.from_buffer_copy(self._buf[i]) for i in range(n.value))
+            if n.value < len(self._buf):
+                return out
+
+    def drain(self):
+        check(lib().cave_serve_drain(self._h))
+        return self.poll()
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        check(lib().cave_serve_stats(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("admitted", "finished", "resident_bytes", "peak_resident_bytes"), (x.value for x in v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cave_serve_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -34,7 +34,7 @@ LIB_SOURCES = [
     "api_common.cpp",
     "api_step.cpp",
     "api_batch.cpp",
-    "api_pending.cpp",
+    "api_serve.cpp",
     "api_search.cpp",
     "batch_plan.cpp",
     "batch.hip",

@@ -20,14 +20,16 @@ struct DevJob {
     uint32_t ticks;        // tick budget
     uint32_t sim_flags;    // CAVE_SIM_*
     uint32_t tier;         // kTierLds, kTierBlockArena or kTierOwnRegion
-    uint32_t result_slot;  // index into results / per-tick rows
-    uint32_t final_slot;   // +1-based index of a final-cells staging area (0 = none)
+    uint32_t result_slot;  // index into results / per-tick rows / completion flags
+    uint32_t has_final;    // 1: store the final cells (reference packed layout) at final_addr
     uint32_t zchunks;      // z-chunks per column per tick over all parts (a multiple of parts)
     uint32_t parts;        // blocks that share the job (1 = one block owns it)
     uint32_t part;         // this queue entry's part: z-chunks [part*c, (part+1)*c), c = zchunks/parts
     uint32_t sync_slot;    // JobSync record of a multi-part job
     uint32_t pad_;
-    uint64_t region;       // word offset of the job's own cells in BatchArgs::job_arena (kTierOwnRegion)
+    uint64_t region;       // device address of the job's own cells (kTierOwnRegion)
+    uint64_t final_addr;   // device address of its final cells (has_final)
+    uint64_t seq;          // served requests: completion value written to BatchArgs::done[result_slot]
     DevInit init;
 };
 
@@ -68,12 +70,34 @@ struct BatchArgs {
     uint32_t per_tick_stride;
     uint32_t *arena;             // global-tier cells: one region per block
     uint64_t arena_words;        // words per block region
-    uint32_t *final_cells;       // staging for final states (reference layout)
-    const uint64_t *final_offsets; // word offset per final slot
     uint32_t lds_words;          // dynamic LDS words available
-    uint32_t *job_arena;         // kTierOwnRegion cells, DevJob::region words in
     JobSync *sync;               // one record per multi-part job
     uint64_t wait_limit;         // device wall-clock ticks a part waits for the others
+    // served requests only (k_serve)
+    const struct ServeCtl *ctl;  // host-coherent: ring tail, stop flag
+    struct ServeDev *sd;         // device-side claim state
+    uint64_t ring_cap;           // ring entries (jobs[] is the host-coherent ring)
+    uint64_t *done;              // host-coherent completion flags, [result_slot]
+    uint64_t idle_limit;         // device wall-clock ticks without a claim before the kernel closes
+};
+
+// Control block of the request ring, in host-coherent memory: the host writes
+// ring entries, then publishes them by raising `tail`; `stop` asks every block
+// to exit once it has no work.
+struct ServeCtl {
+    uint64_t tail;
+    uint64_t stop;
+    uint64_t pad_[6];
+};
+
+// Claim state of the resident kernel, in device memory (zeroed before every
+// launch except `head`, which carries over between launches).
+struct ServeDev {
+    uint64_t head;        // ring entries claimed so far
+    uint64_t tail_seen;   // latest ring tail read from the host
+    uint64_t last_claim;  // device clock of the latest claim
+    uint32_t poll_lock;   // held by the one block reading the host's tail
+    uint32_t closing;     // set once: every block leaves the claim loop
 };
 
 // Words of cell storage a job needs: 4 state planes of ceil(X/32) words per
@@ -105,6 +129,8 @@ inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t thread
 }
 
 const void *batch_kernel();
+const void *serve_kernel();
+int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
 
 }  // namespace cave

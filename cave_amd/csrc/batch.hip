@@ -399,116 +399,236 @@ __device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t ph
     return *s_flag != 0u;
 }
 
+// Block-wide scratch of the batched kernels.
+struct BlockShared {
+    uint32_t sum[3 * 16];
+    int32_t box[6 * 16];
+    uint32_t done;
+    uint32_t ok;
+    uint64_t claim;
+    DevJob job;  // k_serve: the claimed ring entry, copied from host memory
+};
+
+// Runs one queue entry -- one simulation, or one part of a split one -- with
+// the whole block: initial cells, the ticks with the search manager fused in
+// (searchManager.slang:98-149: totals, max / final alive, early death, tick
+// budget), the result record (part 0) and the final cells.  SERVE: the result
+// and completion flag go to host-coherent memory, after every store of the
+// simulation is visible to the host.
+template <bool SERVE>
+__device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
+{
+    const uint64_t t_start = wall_clock64();
+    const JobGeo g = job_geo(j);
+    const RuleConsts rc = rule_consts(j.M, j.born_lut, j.surv_lut);
+    const bool multi = j.parts > 1u;
+    uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
+    uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;
+    if (j.tier == kTierLds) init_cells<false>(j, g, lds, acur);
+    else if (multi) init_cells<true>(j, g, S, acur);
+    else init_cells<false>(j, g, S, acur);
+    CaveGridInfo info{};
+    CaveTickStats last{};
+    const uint32_t cells = j.X * j.Y * j.Z;
+    uint32_t t3[3] = {0u, 0u, 0u};
+    int32_t b3[3] = {0, 0, 0}, c3[3] = {0, 0, 0};
+    bool ok = true;
+    if (multi) ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
+    else __syncthreads();
+    uint32_t ticks_run = 0;  // the same in every thread and every part
+    for (uint32_t t = 0; ok && t < j.ticks; t++) {
+        StatAcc acc;
+        if (j.tier == kTierLds) run_tick<false>(j.flags, g, lds, acur, anext, rc, acc);
+        else if (multi) run_tick<true>(j.flags, g, S, acur, anext, rc, acc);
+        else run_tick<false>(j.flags, g, S, acur, anext, rc, acc);
+        ticks_run++;
+        uint32_t tot[3];
+        int32_t bmin[3], bmax[3];
+        block_sum3(acc.dying, acc.born, acc.surv, sh.sum, tot);
+        if (multi) {
+            // every tick: whether this tick is the last is known only after the totals
+            block_bbox(acc, sh.box, bmin, bmax);
+            ok = part_rendezvous(a, j, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
+            if (!ok) break;
+        }
+        if (threadIdx.x == 0) {
+            const uint32_t dying = tot[0], born = tot[1], surv = tot[2];
+            const uint32_t dead = cells - dying - born - surv;
+            // searchManager.slang:115-139
+            info.numberOfSimulationTicksSurvived++;
+            info.totalNumberOfCellsThatDied += dead;
+            info.totalNumberOfCellsThatWereBorn += born;
+            info.finalAliveCount = surv;
+            info.maxAliveCount = max(info.maxAliveCount, surv);
+            const bool died = (j.sim_flags & CAVE_SIM_LEGACY_DEATH) ? dead == cells : (surv == 0u && born == 0u);
+            if (died) info.status = 1u;
+            else if (info.numberOfSimulationTicksSurvived >= j.ticks) info.status = 2u;
+            else info.status = 0u;
+            last.snap.numberOfDeadCells = dead;
+            last.snap.numberOfDyingCells = dying;
+            last.snap.numberOfCellsBorn = born;
+            last.snap.numberOfSurvivingCells = surv;
+            last.live = born + surv;
+            last.changed = born + dying;
+            const bool stop = (info.status == 1u && !(j.sim_flags & CAVE_SIM_NO_EARLY_EXIT)) || t + 1u >= j.ticks;
+            sh.done = stop ? 1u : 0u;
+        }
+        __syncthreads();
+        // the bounding box is reduced only when someone reads it
+        if (!multi && (sh.done || a.per_tick)) block_bbox(acc, sh.box, bmin, bmax);
+        if (threadIdx.x == 0 && (sh.done || a.per_tick)) {
+            for (int k = 0; k < 3; k++) {
+                last.bbox_min[k] = bmin[k];
+                last.bbox_max[k] = bmax[k];
+            }
+            if (a.per_tick && j.part == 0u) a.per_tick[(size_t)j.result_slot * a.per_tick_stride + t] = last;
+        }
+        const uint32_t tmp = acur;
+        acur = anext;
+        anext = tmp;
+        const bool done = sh.done;
+        __syncthreads();
+        if (done) break;
+    }
+    if (j.has_final && ok) {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(j.final_addr);
+        if (j.tier == kTierLds) store_final(g, lds, dst);
+        else store_final(g, S, dst);
+    }
+    if constexpr (SERVE) {
+        // the final cells of every part reach memory before part 0 reports
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (multi && ok) ok = part_rendezvous(a, j, ticks_run + 1u, t3, b3, c3, t3, b3, c3, &sh.ok);
+        else __syncthreads();
+    }
+    if (threadIdx.x == 0 && j.part == 0u) {
+        DevResult r;
+        r.info = info;
+        // searchSystem.cpp:695-711
+        r.classification = info.status == 2u ? 2u : (info.numberOfSimulationTicksSurvived > j.M + 1u ? 1u : 0u);
+        r.ticks_run = info.numberOfSimulationTicksSurvived;
+        r.last = last;
+        r.t_start = t_start;
+        r.t_end = wall_clock64();
+        r.err = ok ? 0u : 1u;
+        r.pad_ = 0u;
+        a.results[j.result_slot] = r;
+        if constexpr (SERVE) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            __hip_atomic_store(a.done + j.result_slot, j.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_sum[3 * 16];
-    __shared__ int32_t s_box[6 * 16];
-    __shared__ uint32_t s_job;
-    __shared__ uint32_t s_done;
-    __shared__ uint32_t s_ok;
+    __shared__ BlockShared sh;
 
     uint32_t *arena = a.arena ? a.arena + (size_t)blockIdx.x * a.arena_words : nullptr;
     while (true) {
-        if (threadIdx.x == 0) s_job = atomicAdd(a.next_job, 1u);
+        if (threadIdx.x == 0) sh.claim = atomicAdd(a.next_job, 1u);
         __syncthreads();
-        const uint32_t jid = s_job;
+        const uint32_t jid = (uint32_t)sh.claim;
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];
-        const uint64_t t_start = wall_clock64();
-        const JobGeo g = job_geo(j);
-        const RuleConsts rc = rule_consts(j.M, j.born_lut, j.surv_lut);
-        const bool multi = j.parts > 1u;
-        uint32_t *S = j.tier == kTierOwnRegion ? a.job_arena + j.region : arena;
-        uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;
-        if (j.tier == kTierLds) init_cells<false>(j, g, lds, acur);
-        else if (multi) init_cells<true>(j, g, S, acur);
-        else init_cells<false>(j, g, S, acur);
-        CaveGridInfo info{};
-        CaveTickStats last{};
-        const uint32_t cells = j.X * j.Y * j.Z;
-        bool ok = true;
-        if (multi) {
-            uint32_t t3[3] = {0u, 0u, 0u};
-            int32_t b3[3] = {0, 0, 0}, c3[3] = {0, 0, 0};
-            ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &s_ok);
-        } else {
-            __syncthreads();
-        }
-        for (uint32_t t = 0; ok && t < j.ticks; t++) {
-            StatAcc acc;
-            if (j.tier == kTierLds) run_tick<false>(j.flags, g, lds, acur, anext, rc, acc);
-            else if (multi) run_tick<true>(j.flags, g, S, acur, anext, rc, acc);
-            else run_tick<false>(j.flags, g, S, acur, anext, rc, acc);
-            uint32_t tot[3];
-            int32_t bmin[3], bmax[3];
-            block_sum3(acc.dying, acc.born, acc.surv, s_sum, tot);
-            if (multi) {
-                // every tick: whether this tick is the last is known only after the totals
-                block_bbox(acc, s_box, bmin, bmax);
-                ok = part_rendezvous(a, j, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &s_ok);
-                if (!ok) break;
+        run_entry<false>(a, j, lds, arena, sh);
+    }
+}
+
+// Claims the next ring entry for the block (thread 0).  Idle blocks compare
+// the claim counter with the last tail seen on the device; only the block
+// holding the poll lock reads the host's tail over the bus.  That block also
+// closes the kernel -- for every block at once -- when every published entry
+// has been claimed and the host asked to stop, or no claim happened for
+// idle_limit.  Closing only when nothing is left to claim means no part of a
+// split simulation is ever left without its partner blocks.
+__device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
+{
+    ServeDev *sd = a.sd;
+    uint32_t naps = 0;
+    while (true) {
+        if (__hip_atomic_load(&sd->closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        uint64_t h = __hip_atomic_load(&sd->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t seen = __hip_atomic_load(&sd->tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h < seen) {
+            if (__hip_atomic_compare_exchange_strong(&sd->head, &h, h + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(&sd->last_claim, wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                jid = h;
+                return true;
             }
-            if (threadIdx.x == 0) {
-                const uint32_t dying = tot[0], born = tot[1], surv = tot[2];
-                const uint32_t dead = cells - dying - born - surv;
-                // searchManager.slang:115-139
-                info.numberOfSimulationTicksSurvived++;
-                info.totalNumberOfCellsThatDied += dead;
-                info.totalNumberOfCellsThatWereBorn += born;
-                info.finalAliveCount = surv;
-                info.maxAliveCount = max(info.maxAliveCount, surv);
-                const bool died = (j.sim_flags & CAVE_SIM_LEGACY_DEATH) ? dead == cells : (surv == 0u && born == 0u);
-                if (died) info.status = 1u;
-                else if (info.numberOfSimulationTicksSurvived >= j.ticks) info.status = 2u;
-                else info.status = 0u;
-                last.snap.numberOfDeadCells = dead;
-                last.snap.numberOfDyingCells = dying;
-                last.snap.numberOfCellsBorn = born;
-                last.snap.numberOfSurvivingCells = surv;
-                last.live = born + surv;
-                last.changed = born + dying;
-                const bool stop = (info.status == 1u && !(j.sim_flags & CAVE_SIM_NO_EARLY_EXIT)) || t + 1u >= j.ticks;
-                s_done = stop ? 1u : 0u;
-            }
-            __syncthreads();
-            // the bounding box is reduced only when someone reads it
-            if (!multi && (s_done || a.per_tick)) block_bbox(acc, s_box, bmin, bmax);
-            if (threadIdx.x == 0 && (s_done || a.per_tick)) {
-                for (int k = 0; k < 3; k++) {
-                    last.bbox_min[k] = bmin[k];
-                    last.bbox_max[k] = bmax[k];
-                }
-                if (a.per_tick && j.part == 0u) a.per_tick[(size_t)j.result_slot * a.per_tick_stride + t] = last;
-            }
-            const uint32_t tmp = acur;
-            acur = anext;
-            anext = tmp;
-            const bool done = s_done;
-            __syncthreads();
-            if (done) break;
+            continue;
         }
-        if (threadIdx.x == 0 && j.part == 0u) {
-            DevResult r;
-            r.info = info;
-            // searchSystem.cpp:695-711
-            r.classification = info.status == 2u ? 2u : (info.numberOfSimulationTicksSurvived > j.M + 1u ? 1u : 0u);
-            r.ticks_run = info.numberOfSimulationTicksSurvived;
-            r.last = last;
-            r.t_start = t_start;
-            r.t_end = wall_clock64();
-            r.err = ok ? 0u : 1u;
-            r.pad_ = 0u;
-            a.results[j.result_slot] = r;
+        uint32_t unlocked = 0u;
+        if (__hip_atomic_compare_exchange_strong(&sd->poll_lock, &unlocked, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            const uint64_t tail = __hip_atomic_load(&a.ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const uint64_t stop = __hip_atomic_load(&a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(&sd->tail_seen, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t now_head = __hip_atomic_load(&sd->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool close = false;
+            if (now_head >= tail) {  // every published entry is claimed
+                const uint64_t last = __hip_atomic_load(&sd->last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                close = stop != 0u || wall_clock64() - last > a.idle_limit;
+            }
+            if (close) __hip_atomic_store(&sd->closing, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sd->poll_lock, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (close) return false;
+            if (now_head < tail) continue;
         }
-        if (j.final_slot && ok) {
-            uint32_t *dst = a.final_cells + a.final_offsets[j.final_slot - 1u];
-            if (j.tier == kTierLds) store_final(g, lds, dst);
-            else store_final(g, S, dst);
+        // back off: short naps while work may be arriving, longer ones after
+        if (naps++ < 16u) __builtin_amdgcn_s_sleep(4);
+        else __builtin_amdgcn_s_sleep(64);
+    }
+}
+
+// Persistent server: blocks claim ring entries (host-coherent memory) as the
+// host publishes them, run them with the batched kernel's code, and report
+// each result with a completion flag.  Every block exits once the kernel is
+// closed (see serve_claim).
+__global__ void __launch_bounds__(1024) k_serve(BatchArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ BlockShared sh;
+
+    constexpr uint32_t kJobWords = sizeof(DevJob) / 4u;
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint64_t jid = 0;
+            sh.claim = serve_claim(a, jid) ? jid : ~0ull;
         }
         __syncthreads();
+        const uint64_t jid = sh.claim;
+        if (jid == ~0ull) break;
+        // the entry lives in host memory: one load per word, then every
+        // thread takes the words from LDS into scalar registers
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(a.jobs + jid % a.ring_cap);
+        if (threadIdx.x < kJobWords)
+            reinterpret_cast<uint32_t *>(&sh.job)[threadIdx.x] =
+                __hip_atomic_load(src + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
+        DevJob j;
+#pragma unroll
+        for (uint32_t k = 0; k < kJobWords; k++)
+            reinterpret_cast<uint32_t *>(&j)[k] =
+                __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t *>(&sh.job)[k]);
+        __syncthreads();
+        run_entry<true>(a, j, lds, nullptr, sh);
     }
+}
+
+const void *serve_kernel() { return (const void *)k_serve; }
+
+int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st)
+{
+    if (lds_bytes > 64u * 1024u)
+        CAVE_HIP(hipFuncSetAttribute((const void *)k_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    hipLaunchKernelGGL(k_serve, dim3(blocks), dim3(threads), lds_bytes, st, a);
+    CAVE_HIP(hipGetLastError());
+    return CAVE_OK;
 }
 
 const void *batch_kernel() { return (const void *)k_batch; }

@@ -75,21 +75,6 @@ int dev_alloc(DevBuf &b, size_t bytes)
     return CAVE_OK;
 }
 
-uint64_t wall_clock_khz()
-{
-    static int khz = 0;
-    if (!khz && (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0))
-        khz = 100000;
-    return (uint64_t)khz;
-}
-
-// LDS a block may use so that 1024 / threads blocks share one CU.
-uint32_t lds_cap(uint32_t threads)
-{
-    const uint32_t per = std::max(1u, 1024u / threads);
-    return kLdsBytesPerCU / per - kStaticLds;
-}
-
 double split_divisor()
 {
     const char *e = std::getenv("CAVE_SPLIT_DIV");
@@ -152,6 +137,37 @@ struct BucketWork {
 };
 
 }  // namespace
+
+uint64_t wall_clock_khz()
+{
+    static int khz = 0;
+    if (!khz && (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0))
+        khz = 100000;
+    return (uint64_t)khz;
+}
+
+uint32_t lds_cap(uint32_t threads)
+{
+    const uint32_t per = std::max(1u, 1024u / threads);
+    return kLdsBytesPerCU / per - kStaticLds;
+}
+
+double bucket_rate(uint32_t bucket, uint32_t flags, uint32_t threads, uint32_t ipt, const uint32_t dims[3],
+                   bool measure, hipStream_t st)
+{
+    const TuneKey key{bucket, flags & 7u, threads, ipt};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_rates.find(key);
+        if (it != g_rates.end()) return it->second;
+    }
+    if (!measure || !tuner_enabled()) return 0.0;
+    double r = 0;
+    if (calibrate(key, dims, st, r)) return 0.0;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_rates[key] = r;
+    return r;
+}
 
 uint32_t cu_count()
 {
@@ -225,8 +241,8 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
         b.work += cost[i];
         if (specs[i].flags & CAVE_SIM_FINAL_HASH) {
-            plan.final_offsets.push_back(plan.final_words);
-            j.final_slot = (uint32_t)plan.final_offsets.size();
+            j.has_final = 1u;
+            j.final_addr = plan.final_words;  // word offset until run_plan places the staging
             plan.final_words += cave_cell_words(j.X, j.Y, j.Z);
         }
     }
@@ -351,7 +367,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         plan.ipt[i] = ipt_of[i];
         if (parts > 1) {
             j.tier = kTierOwnRegion;
-            j.region = plan.region_words;
+            j.region = plan.region_words;  // word offset until run_plan places the regions
             plan.region_words += (words[i] + 63u) / 64u * 64u;
             j.sync_slot = plan.nsync++;
         } else if (j.tier == kTierBlockArena) {
@@ -381,7 +397,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
 int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
 {
     const size_t nq = plan.queue.size();
-    DevBuf djobs, dnext, dres, dpt, darena, dfinal, dfoff, dregion, dsync;
+    DevBuf djobs, dnext, dres, dpt, darena, dfinal, dregion, dsync;
     int rc;
     if ((rc = dev_alloc(djobs, sizeof(DevJob) * nq))) return rc;
     if ((rc = dev_alloc(dnext, sizeof(uint32_t)))) return rc;
@@ -395,13 +411,14 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
         if ((rc = dev_alloc(dsync, sizeof(JobSync) * plan.nsync))) return rc;
         CAVE_HIP(hipMemsetAsync(dsync.p, 0, sizeof(JobSync) * plan.nsync, st));
     }
-    if (plan.final_words) {
-        if ((rc = dev_alloc(dfinal, sizeof(uint32_t) * plan.final_words))) return rc;
-        if ((rc = dev_alloc(dfoff, sizeof(uint64_t) * plan.final_offsets.size()))) return rc;
-        CAVE_HIP(hipMemcpyAsync(dfoff.p, plan.final_offsets.data(), sizeof(uint64_t) * plan.final_offsets.size(),
-                                hipMemcpyHostToDevice, st));
+    if (plan.final_words && (rc = dev_alloc(dfinal, sizeof(uint32_t) * plan.final_words))) return rc;
+    // word offsets -> device addresses
+    std::vector<DevJob> queue = plan.queue;
+    for (DevJob &e : queue) {
+        if (e.tier == kTierOwnRegion) e.region = (uint64_t)(dregion.as<uint32_t>() + e.region);
+        if (e.has_final) e.final_addr = (uint64_t)(dfinal.as<uint32_t>() + e.final_addr);
     }
-    CAVE_HIP(hipMemcpyAsync(djobs.p, plan.queue.data(), sizeof(DevJob) * nq, hipMemcpyHostToDevice, st));
+    CAVE_HIP(hipMemcpyAsync(djobs.p, queue.data(), sizeof(DevJob) * nq, hipMemcpyHostToDevice, st));
     CAVE_HIP(hipMemsetAsync(dnext.p, 0, sizeof(uint32_t), st));
 
     BatchArgs a{};
@@ -413,10 +430,7 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
     a.per_tick_stride = stride;
     a.arena = darena.as<uint32_t>();
     a.arena_words = plan.arena_words;
-    a.final_cells = dfinal.as<uint32_t>();
-    a.final_offsets = dfoff.as<uint64_t>();
     a.lds_words = plan.lds_bytes / 4u;
-    a.job_arena = dregion.as<uint32_t>();
     a.sync = dsync.as<JobSync>();
     a.wait_limit = wall_clock_khz() * 1000u * kPartWaitSeconds;
 
@@ -470,9 +484,8 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
         }
     }
     for (const DevJob &j : plan.jobs)
-        if (j.final_slot)
-            results[j.result_slot].final_hash =
-                fnv_ref_layout(hfinal.data() + plan.final_offsets[j.final_slot - 1], j.X, j.Y, j.Z);
+        if (j.has_final)
+            results[j.result_slot].final_hash = fnv_ref_layout(hfinal.data() + j.final_addr, j.X, j.Y, j.Z);
     return CAVE_OK;
 }
 

@@ -27,8 +27,7 @@ struct BatchPlan {
     uint64_t arena_words = 0;   // per block (kTierBlockArena)
     uint64_t region_words = 0;  // total of kTierOwnRegion regions
     uint32_t nsync = 0;
-    std::vector<uint64_t> final_offsets;
-    uint64_t final_words = 0;
+    uint64_t final_words = 0;   // staging for final cells (CAVE_SIM_FINAL_HASH)
     double predicted_ms = 0;    // the tuner's estimate of the launch (0 without tuning)
 };
 
@@ -43,6 +42,18 @@ uint32_t cu_count();
 DevJob make_job(const CaveSimSpec &s, uint32_t slot);
 
 int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPlan &plan);
+
+// LDS a block of `threads` threads may use so that 1024 / threads blocks share a CU.
+uint32_t lds_cap(uint32_t threads);
+
+// Device-wide rate (cell-updates/s, every resident block on the bucket) of a
+// (grid-edge bucket, neighbourhood) at a block size and items per thread from
+// the tuner's table.  When missing and `measure` is set, one calibration
+// launch on `st` with a grid of `dims` measures it; 0 when unknown.
+double bucket_rate(uint32_t bucket, uint32_t flags, uint32_t threads, uint32_t ipt, const uint32_t dims[3],
+                   bool measure, hipStream_t st);
+
+uint64_t wall_clock_khz();
 
 // Runs a planned batch on `st` and fills results (and the options' per-tick
 // rows, timing and device time).

@@ -285,25 +285,36 @@ int cave_search_destroy(CaveSearch *h);
 typedef struct CaveServer CaveServer;
 
 typedef struct CaveServeOptions {
-    uint64_t hbm_budget_bytes; /* admission budget for resident state */
-    uint32_t max_slots;        /* concurrent resident requests (0 = auto) */
-    uint32_t slice_ticks;      /* ticks per scheduling round (0 = auto) */
+    uint64_t hbm_budget_bytes; /* admission budget for resident cell state + final-cell staging (0 = 200 GB) */
+    uint32_t max_inflight;     /* requests admitted and not yet finished (0 = 16384) */
+    uint32_t split_target_us;  /* a request whose one-block run is estimated longer is split (0 = 4000) */
+    uint32_t idle_exit_us;     /* the resident kernel closes after this long without a claim (0 = 50000) */
+    uint32_t reserved;
 } CaveServeOptions;
 
 typedef struct CaveServeCompletion {
     uint64_t request_id;
     CaveSimResult result;
-    double submit_s;  /* host clock, seconds since server start */
-    double admit_s;
-    double finish_s;
+    double submit_s; /* host clock, seconds since cave_serve_create */
+    double admit_s;  /* its state was reserved and it was published to the device */
+    double finish_s; /* the host saw its completion */
+    double run_ms;   /* device time from its first claim to its result */
+    uint32_t blocks; /* blocks it ran on */
+    int32_t status;  /* CAVE_OK, or CAVE_E_STATE when its parts did not meet in time */
 } CaveServeCompletion;
 
+/* Continuous batching: a resident persistent kernel claims requests from a
+ * ring in host-coherent memory as the host publishes them, so requests that
+ * arrive while others run are admitted (HBM budget, in-flight cap) and start
+ * on the next free block.  A request's packed state stays in its own HBM
+ * region from admission to completion.  One handle per host thread. */
 int cave_serve_create(const CaveServeOptions *opt, CaveServer **out);
-/* Queues a request; CAVE_E_BUDGET if it can never fit the budget. */
+/* Queues a request (admitted in arrival order); CAVE_E_BUDGET if it can never
+ * fit the budget. */
 int cave_serve_submit(CaveServer *s, const CaveSimSpec *spec, uint64_t request_id);
-/* Moves finished requests out (non-blocking). */
+/* Admits what fits, then moves finished requests out (non-blocking). */
 int cave_serve_poll(CaveServer *s, CaveServeCompletion *out, size_t cap, size_t *n);
-/* Blocks until every submitted request finished. */
+/* Blocks until every submitted request finished (completions stay pollable). */
 int cave_serve_drain(CaveServer *s);
 int cave_serve_stats(const CaveServer *s, uint64_t *admitted, uint64_t *finished,
                      uint64_t *resident_bytes, uint64_t *peak_resident_bytes);

@@ -36,6 +36,13 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(cave_amd.GridInfo) == 24
     assert C.sizeof(cave_amd.ChunkConfiguration) == 56
     assert C.sizeof(cave_amd.TickStats) == 48
+    # this library's own records (no reference counterpart)
+    assert C.sizeof(cave_amd.SimSpec) == 72
+    assert C.sizeof(cave_amd.SimResult) == 88
+    assert C.sizeof(cave_amd.BatchOptions) == 40
+    assert cave_amd.TIMING_DTYPE.itemsize == 24
+    assert C.sizeof(cave_amd.ServeOptions) == 24
+    assert C.sizeof(cave_amd.ServeCompletion) == 136
 
 
 def test_abi_version():
