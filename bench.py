@@ -1,6 +1,6 @@
 """Benchmark of the CAVE MI355X engine.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-search]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-search] [--no-serve]
 
 A step is one tick of the single-grid workload: a 512^3 grid, FEV
 neighbourhood (26 neighbours), C = 8 states (M = 7), random birth/survival
@@ -10,8 +10,10 @@ before the timed region.  `value` is cell-updates/s over the K timed ticks.
 
 The same JSON line also carries the rule-search rate at 49^3 x 20 ticks
 (4096 configs x 32 seeds, FE, C in 2..5), the uneven-size autotuned search
-(2048 configs, edge 24..160) with its per-bucket achieved fractions, and the
-host-CPU baselines, each timed on a bounded sample with the CPU oracle.
+(2048 configs, edge 24..160) with its per-bucket achieved fractions, online
+serving (Poisson 2000 requests/s for 60 s into the resident kernel: throughput,
+p50 / p99 latency, oracle spot checks) and the host-CPU baselines, each timed
+on a bounded sample with the CPU oracle.
 """
 from __future__ import annotations
 
@@ -251,6 +253,116 @@ def run_uneven_bench(n_configs=2048, cpu=True):
     return out
 
 
+def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True):
+    """Online serving: Poisson arrivals at rate_hz for duration_s, each request
+    a grid of edge 32..192 (log-uniform), F/FE/FEV, C in 2..16, bits p = 0.12,
+    20..500 ticks, admitted against a 200 GB HBM budget into the resident
+    persistent kernel.  Latency = completion seen by the host - submission.
+    1% of the requests (every 100th) carry a final-cell hash and are checked
+    against the CPU oracle, cheapest first, within a bounded CPU budget."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    import cave_amd as cv
+    from cave_amd import workloads as W
+
+    times, specs = W.serving_requests(rate_hz, duration_s)
+    n = len(specs)
+    sample = np.arange(0, n, 100)
+    specs["sim_flags"][sample] |= cv.SIM_FINAL_HASH
+    specs = np.ascontiguousarray(specs)
+    srv = cv.Server(hbm_budget_bytes=200 << 30)
+    for i in range(8):  # warm-up requests (code objects, first arena chunk)
+        srv.submit_row(specs, i, 1 << 40 | i)
+    srv.drain()
+    log(f"  serving {n} requests over {duration_s:.0f} s")
+    done = {}
+    t0 = time.perf_counter()
+    nxt, last_log = 0, 0.0
+    while len(done) < n:
+        now = time.perf_counter() - t0
+        while nxt < n and times[nxt] <= now:
+            srv.submit_row(specs, nxt, nxt)
+            nxt += 1
+        for c in srv.poll():
+            done[c.request_id] = c
+        if now - last_log >= 10.0:
+            last_log = now
+            log(f"  t={now:5.1f} s submitted {nxt} finished {len(done)}")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    cells = specs["dims"].astype(np.int64).prod(1)
+    ticks_run = np.array([done[i].result.ticks_run for i in range(n)], dtype=np.int64)
+    lat = np.array([done[i].finish_s - done[i].submit_s for i in range(n)]) * 1e3
+    queue = np.array([done[i].admit_s - done[i].submit_s for i in range(n)]) * 1e3
+    run = np.array([done[i].run_ms for i in range(n)])
+    blocks = np.array([done[i].blocks for i in range(n)])
+    bad = sum(done[i].status != 0 for i in range(n))
+    ups = int((ticks_run * cells).sum())
+    out = {
+        "metric": "served rule evaluations: throughput and p50/p99 latency",
+        "value": n / wall, "unit": "requests/s", "offered_rate": rate_hz, "requests": n, "wall_s": wall,
+        "cell_updates_per_s": ups / wall,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max()), "mean": float(lat.mean())},
+        "admission_wait_ms": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+        "device_run_ms": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
+        "split_requests": int((blocks > 1).sum()), "failed_requests": int(bad),
+        "peak_resident_bytes": st["peak_resident_bytes"], "hbm_budget_bytes": 200 << 30,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        def check(i):
+            sp = specs[i]
+            X, Y, Z = (int(v) for v in sp["dims"])
+            init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
+                                float(sp["density"]), int(sp["seed"]))
+            t, info, _, final = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
+                                           int(sp["survival"]), int(sp["ticks"]), init)
+            r = done[i].result
+            same = (r.ticks_run == t and r.info.finalAliveCount == info.finalAliveCount and
+                    r.info.totalNumberOfCellsThatDied == info.totalNumberOfCellsThatDied and
+                    r.info.totalNumberOfCellsThatWereBorn == info.totalNumberOfCellsThatWereBorn and
+                    r.final_hash == O.canonical_hash(final, X, Y, Z))
+            return same, t * X * Y * Z
+
+        workers = min(16, os.cpu_count() or 1)
+        order = sample[np.argsort(ticks_run[sample] * cells[sample], kind="stable")]
+        # a request above 2e8 executed cell-updates takes the scalar oracle > 15 s
+        eligible = [int(i) for i in order if ticks_run[i] * cells[i] <= 2e8]
+        checked, mism, cpu_ups = 0, 0, 0
+        t1 = time.perf_counter()
+        with cf.ThreadPoolExecutor(workers) as ex:
+            it = iter(eligible)
+            running = set()
+            for i in it:
+                running.add(ex.submit(check, i))
+                if len(running) >= workers:
+                    break
+            while running:
+                fin, running = cf.wait(running, return_when=cf.FIRST_COMPLETED)
+                for f in fin:
+                    same, u = f.result()
+                    checked += 1
+                    mism += 0 if same else 1
+                    cpu_ups += u
+                    if time.perf_counter() - t1 < 20.0:
+                        i = next(it, None)
+                        if i is not None:
+                            running.add(ex.submit(check, i))
+        out["spot_checks"] = {"sampled": int(len(sample)), "checked": checked, "mismatches": mism,
+                              "method": "every 100th request carries a final-cell hash; the CPU oracle reruns "
+                                        f"the sampled requests cheapest first for 20 s on {workers} threads "
+                                        "(requests above 2e8 executed cell-updates are not rerun)",
+                              "cpu_cell_updates_per_s": cpu_ups / (time.perf_counter() - t1), "cores": workers}
+        assert mism == 0, f"{mism} served results differ from the CPU oracle"
+    return out
+
+
 def copy_calibration(mib: int, copies: int):
     """K plain device-to-device copies of a MIB-MiB buffer: each reads and
     writes exactly MIB MiB, the yardstick for the HBM byte counters."""
@@ -275,6 +387,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-configs", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run K device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
@@ -330,6 +445,9 @@ def main():
         out["search"] = run_search_bench(a.search_configs, cpu=not a.no_cpu)
         log("rule search, uneven sizes: 2048 configs")
         out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
+    if not a.no_serve:
+        log(f"serving: Poisson {a.serve_rate:.0f} req/s for {a.serve_seconds:.0f} s")
+        out["serving"] = run_serving_bench(a.serve_rate, a.serve_seconds, cpu=not a.no_cpu)
     log("done")
     print(json.dumps(out), flush=True)
 

@@ -520,6 +520,11 @@ class Server:
             spec = SimSpec.from_buffer_copy(rec.tobytes())
         check(lib().cave_serve_submit(self._h, C.byref(spec), request_id))
 
+    def submit_row(self, specs: np.ndarray, i: int, request_id: int) -> None:
+        """Submits row i of a contiguous SIM_DTYPE array (no copy)."""
+        ptr = C.cast(specs.ctypes.data + i * SIM_DTYPE.itemsize, C.POINTER(SimSpec))
+        check(lib().cave_serve_submit(self._h, ptr, request_id))
+
     def poll(self):
         out = []
         n = C.c_size_t(0)
