@@ -11,7 +11,7 @@
  * a plain, per-cell, literal restatement: no SWAR, no tiling, no shortcuts, so
  * that it stays easy to audit against the Slang/C++ text.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
+ * Pinning (what checks each function; tests/test_oracle.py):
  *   orc_rule_string_to_bitmask / orc_bitmask_to_rule_string
  *       -- cli-rules KAT (src/testing/testRunner.cpp:231-260).
  *   orc_enumerate_valid_bitmasks
@@ -20,12 +20,15 @@
  *   orc_search_total_permutations
  *       -- 71,532 = CSV Total (benchmarks/reference/debug/bench_results.csv:2-9).
  *   orc_step (transition + snapshot)
- *       -- brute force on tiny grids against an independent 27-offset
- *          formulation, invariants (bucket sum, dead grid stays dead, dying
+ *       -- an independent numpy model (27-offset formulation), brute force on
+ *          tiny grids, invariants (bucket sum, dead grid stays dead, dying
  *          decrements, padding stays 0, single-cell birth of 6/18/26 neighbours).
  *   orc_run_search (manager + classification + init + step together)
- *       -- the reference's published integration counts at 25^3
- *          (848 / 29,032 / 41,652) from bench_results.csv:2-5.
+ *       -- the reference's 49^3 integration counts (bench_results.csv:6-9):
+ *          under the superseded death test (manager.comp:115) the unviable
+ *          count (24,732) and the viable + possible total match exactly.
+ *          Parity unpinned: the viable / possible split of 248 rules, and the
+ *          25^3 CSV row, which neither death test reproduces.
  *   orc_init_cells
  *       -- textbook Thomas Wang 32-bit hash vectors + spawn-box closed form.
  */
