@@ -20,7 +20,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cave.h")
 
 CAVE_OK, CAVE_E_INVALID, CAVE_E_UNSUPPORTED, CAVE_E_NOMEM, CAVE_E_DEVICE, CAVE_E_BUDGET, CAVE_E_STATE = range(7)
 FACES, EDGES, CORNERS, WRAP = 1, 2, 4, 8
-SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER = 0, 1, 2
+SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER, SPAWN_RENDER_RANDOM = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE = 0, 1, 2, 3
 ENGINE_AUTO, ENGINE_PERSISTENT, ENGINE_PLAIN = 0, 1, 2
 SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH = 1, 2, 4

@@ -57,7 +57,8 @@ int validate_sim(const CaveSimSpec &s)
     if (s.neighborhood_flags & CAVE_WRAP) return fail(CAVE_E_UNSUPPORTED, "wrap (W) is not supported");
     if (s.neighborhood_flags & ~7u) return fail(CAVE_E_INVALID, "bad neighborhood flags");
     if (s.ticks == 0) return fail(CAVE_E_INVALID, "ticks must be >= 1");
-    if (s.spawn_mode > CAVE_SPAWN_FILLED) return fail(CAVE_E_INVALID, "spawn_mode must be random or filled");
+    if (s.spawn_mode == CAVE_SPAWN_BUFFER || s.spawn_mode > CAVE_SPAWN_RENDER_RANDOM)
+        return fail(CAVE_E_INVALID, "spawn_mode must be random, filled or render-random");
     return CAVE_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <sstream>
@@ -120,6 +121,18 @@ uint32_t density_threshold(double density)
     return density >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(density * 4294967296.0);
 }
 
+uint32_t render_density_threshold(double density, bool *all_alive)
+{   // the density arrives as the std::to_string ("%f") text of a float
+    // (simulation.h:19, simulationRenderer.cpp:201); cellInit.slang:57-58
+    char text[64];
+    std::snprintf(text, sizeof text, "%f", (double)(float)density);
+    const float d = std::strtof(text, nullptr);
+    *all_alive = d >= 1.0f;
+    if (*all_alive) return 0u;
+    const float t = d * 4294967296.0f;
+    return t <= 0.0f ? 0u : (uint32_t)t;
+}
+
 DevInit make_init(const uint32_t dims[3], const uint32_t spawn[3], uint32_t spawn_mode, double density,
                   uint32_t seed)
 {   // searchSystem.cpp:264-279: box [(G-S)/2, (G+S)/2) per axis, S clamped to G
@@ -129,7 +142,15 @@ DevInit make_init(const uint32_t dims[3], const uint32_t spawn[3], uint32_t spaw
         in.box_lo[a] = (dims[a] - s) / 2;
         in.box_hi[a] = (dims[a] + s) / 2;
     }
-    in.random = spawn_mode == CAVE_SPAWN_RANDOM ? 1u : 0u;
+    in.random = spawn_mode == CAVE_SPAWN_RANDOM || spawn_mode == CAVE_SPAWN_RENDER_RANDOM ? 1u : 0u;
+    if (spawn_mode == CAVE_SPAWN_RENDER_RANDOM) {
+        // cellInit.slang:44-60: the search hash with seed 0, a float threshold
+        bool all = false;
+        in.threshold = render_density_threshold(density, &all);
+        in.all_in = all ? 1u : 0u;
+        in.seed = 0;
+        return in;
+    }
     in.all_in = density >= 1.0 ? 1u : 0u;
     in.threshold = density_threshold(density);
     in.seed = seed;

@@ -4,7 +4,12 @@
 // ComputeSystem::Tick, src/vulkan/systems/computeSystem.cpp:144-395) with a
 // HIP stream owning two resident cell buffers (pointer-swap ping-pong, no
 // copy-back as in searchSystem.cpp:599-616).  The transition is the search
-// kernel's (searchSimulationCube.slang:78-247): non-wrapping, W rejected.
+// kernel's (searchSimulationCube.slang:78-247), which the render path's
+// (rayMarchCompute.slang:150-178) equals on non-wrapping grids; with the W
+// flag every axis wraps (rayMarchCompute.slang:80-108,
+// simulationRenderer.cpp:595) and the wrapping gather kernel runs.  With
+// CAVE_STEP_GRAPHS=1 a run of ticks is one HIP graph of step launches
+// (captured once per chunk length and buffer parity, replayed after).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +30,7 @@ struct CaveStep {
     DevRule rule{};
     CaveSimulationParameters params{};
     int kernel = CAVE_KERNEL_GATHER;
+    bool wrap = false;
     StreamPlan plan{};
     uint32_t nparts = 0;
     hipStream_t stream = nullptr;
@@ -33,18 +39,24 @@ struct CaveStep {
     StatPart *partials = nullptr;   // [chunk ticks][nparts]
     CaveTickStats *dstats = nullptr; // [chunk ticks]
     uint32_t chunk = 0;
+    // instantiated tick loops: one graph per (ticks, which buffer is current)
+    struct TickGraph {
+        uint32_t ticks;
+        const uint32_t *cur;
+        hipGraphExec_t exec;
+    };
+    std::vector<TickGraph> graphs;
+    bool use_graphs = false;
 };
 
 namespace {
 
 constexpr uint32_t kChunkTicks = 1024;
 
-int parse_rule(const CaveSimulationParameters &p, DevRule *out)
+int parse_rule(const CaveSimulationParameters &p, DevRule *out, bool *wrap)
 {
     const uint32_t flags = uint32_t(p.survivalAndNeighborhoodRules >> 60);
-    if (flags & CAVE_WRAP)
-        return fail(CAVE_E_UNSUPPORTED, "wrap (W) is not supported by the cube search step "
-                                        "(searchSimulationCube.slang:26-29)");
+    *wrap = (flags & CAVE_WRAP) != 0;
     if ((p.shapeAndGridConfiguration & 0x1Fu) != 0)
         return fail(CAVE_E_UNSUPPORTED, "only the cube shape (id 0) is supported");
     const uint32_t M = uint32_t(p.birthAndMaxCellStateRules >> 60) + 1u;
@@ -52,9 +64,57 @@ int parse_rule(const CaveSimulationParameters &p, DevRule *out)
     return CAVE_OK;
 }
 
+void drop_graphs(CaveStep *h)
+{
+    for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
+// Launches the n step kernels of a chunk (ticks t write partials row t).
+int launch_ticks(CaveStep *h, uint32_t n, uint32_t *cur, uint32_t *nxt)
+{
+    for (uint32_t t = 0; t < n; t++) {
+        StatPart *parts = h->partials + (size_t)t * h->nparts;
+        int rc = h->kernel != CAVE_KERNEL_GATHER
+                     ? launch_stream(cur, nxt, h->g, h->rule, h->plan, parts, h->stream)
+                     : launch_gather(cur, nxt, h->g, h->rule, parts, h->stream, nullptr, h->wrap);
+        if (rc) return rc;
+        std::swap(cur, nxt);
+    }
+    return CAVE_OK;
+}
+
+// The chunk's launches as one instantiated graph, captured on first use.
+int tick_graph(CaveStep *h, uint32_t n, hipGraphExec_t *exec)
+{
+    for (const auto &g : h->graphs)
+        if (g.ticks == n && g.cur == h->cur) {
+            *exec = g.exec;
+            return CAVE_OK;
+        }
+    if (h->graphs.size() >= 8) drop_graphs(h);
+    hipGraph_t graph = nullptr;
+    CAVE_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_ticks(h, n, h->cur, h->nxt);
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(CAVE_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(CAVE_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    h->graphs.push_back({n, h->cur, x});
+    *exec = x;
+    return CAVE_OK;
+}
+
 void release(CaveStep *h)
 {
     if (!h) return;
+    drop_graphs(h);
     if (h->cur) (void)hipFree(h->cur);
     if (h->nxt) (void)hipFree(h->nxt);
     if (h->partials) (void)hipFree(h->partials);
@@ -81,14 +141,14 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     const uint64_t cells = (uint64_t)spec->dims[0] * spec->dims[1] * spec->dims[2];
     if (cells > 0xFFFFFFFFull)
         return fail(CAVE_E_INVALID, "grid larger than 2^32 cells: per-tick counters are u32");
-    if (spec->spawn_mode > CAVE_SPAWN_BUFFER) return fail(CAVE_E_INVALID, "bad spawn_mode");
+    if (spec->spawn_mode > CAVE_SPAWN_RENDER_RANDOM) return fail(CAVE_E_INVALID, "bad spawn_mode");
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER && !spec->initial_cells)
         return fail(CAVE_E_INVALID, "CAVE_SPAWN_BUFFER needs initial_cells");
     if (spec->kernel > CAVE_KERNEL_STREAM_NIBBLE) return fail(CAVE_E_INVALID, "bad kernel selector");
 
     CaveStep *h = new (std::nothrow) CaveStep();
     if (!h) return fail(CAVE_E_NOMEM, "host allocation failed");
-    int rc = parse_rule(spec->params, &h->rule);
+    int rc = parse_rule(spec->params, &h->rule, &h->wrap);
     if (rc) { release(h); return rc; }
     h->params = spec->params;
     h->g.X = spec->dims[0];
@@ -97,11 +157,12 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
 
     GridGeom sg = h->g;
     sg.pitch = 4u * ((h->g.X + 31u) / 32u);
-    const bool stream_ok = stream_supported(sg);
+    const bool stream_ok = stream_supported(sg) && !h->wrap;
     const bool want_stream = spec->kernel == CAVE_KERNEL_STREAM || spec->kernel == CAVE_KERNEL_STREAM_NIBBLE;
     if (want_stream && !stream_ok) {
         release(h);
-        return fail(CAVE_E_UNSUPPORTED, "the streaming kernel needs X <= 512");
+        return fail(CAVE_E_UNSUPPORTED, h->wrap ? "wrapping grids run on the gather kernel"
+                                                : "the streaming kernel needs X <= 512");
     }
     if (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) h->kernel = CAVE_KERNEL_GATHER;
     else h->kernel = spec->kernel == CAVE_KERNEL_STREAM_NIBBLE ? CAVE_KERNEL_STREAM_NIBBLE : CAVE_KERNEL_STREAM;
@@ -135,6 +196,11 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     CK(hipMalloc(&h->nxt, bytes));
     CK(hipMemsetAsync(h->nxt, 0, bytes, h->stream));
     h->chunk = kChunkTicks;
+    // CAVE_STEP_GRAPHS=1 replays each run of ticks as a captured graph.  Off by
+    // default: the eager launches already run back to back on the device (the
+    // same device time per tick at 49^3 .. 512^3), and a capture costs host
+    // time the first time a run length is seen.
+    if (const char *env = std::getenv("CAVE_STEP_GRAPHS")) h->use_graphs = env[0] == '1';
     CK(hipMalloc(&h->partials, sizeof(StatPart) * (size_t)h->nparts * h->chunk));
     CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
@@ -160,15 +226,19 @@ int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *
     uint32_t done = 0;
     while (done < ticks) {
         const uint32_t n = std::min(ticks - done, h->chunk);
-        CAVE_HIP(hipEventRecord(h->ev0, h->stream));
-        for (uint32_t t = 0; t < n; t++) {
-            StatPart *parts = h->partials + (size_t)t * h->nparts;
-            int rc = h->kernel != CAVE_KERNEL_GATHER
-                         ? launch_stream(h->cur, h->nxt, h->g, h->rule, h->plan, parts, h->stream)
-                         : launch_gather(h->cur, h->nxt, h->g, h->rule, parts, h->stream);
+        hipGraphExec_t exec = nullptr;
+        if (h->use_graphs) {
+            int rc = tick_graph(h, n, &exec);
             if (rc) return rc;
-            std::swap(h->cur, h->nxt);
         }
+        CAVE_HIP(hipEventRecord(h->ev0, h->stream));
+        if (exec) {
+            CAVE_HIP(hipGraphLaunch(exec, h->stream));
+        } else {
+            int rc = launch_ticks(h, n, h->cur, h->nxt);
+            if (rc) return rc;
+        }
+        if (n & 1u) std::swap(h->cur, h->nxt);
         CAVE_HIP(hipEventRecord(h->ev1, h->stream));
         int rc = launch_reduce_ticks(h->partials, h->nparts, n, h->dstats, h->stream);
         if (rc) return rc;
@@ -227,8 +297,12 @@ int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params)
 {
     if (!h || !params) return fail(CAVE_E_INVALID, "null argument");
     DevRule r{};
-    int rc = parse_rule(*params, &r);
+    bool wrap = false;
+    int rc = parse_rule(*params, &r, &wrap);
     if (rc) return rc;
+    if (wrap != h->wrap)
+        return fail(CAVE_E_STATE, "the W flag cannot change on a live handle (it selects the kernel)");
+    drop_graphs(h);  // the rule is baked into the captured launches
     h->rule = r;
     h->params = *params;
     return CAVE_OK;

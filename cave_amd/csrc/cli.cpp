@@ -101,8 +101,10 @@ const char *kHelp =
     "          --spawn-mode random|filled, --spawn-density d | --density-sweep a:b:s,\n"
     "          --seed-runs R, --output DIR, --legacy-death\n"
     "          (accepted and ignored: --headless --dual-gpu --search-workgroup-size --gpu-vram-budget-gb --shape cube)\n"
-    "  step:   --grid N, --birth \"5,6\", --survival \"4-6\", --max-cs M, --neighborhood FE, --frames T,\n"
-    "          --spawn S, --spawn-mode random|filled, --spawn-density d, --seed s, --dump FILE\n";
+    "  step:   --grid N, --birth \"5,6\", --survival \"4-6\", --max-cs M, --neighborhood FE[W], --frames T,\n"
+    "          --spawn S, --spawn-mode random|filled, --spawn-density d, --seed s, --dump FILE\n"
+    "          (--mode render --headless: the same loop with the render path's unsalted random spawn;\n"
+    "           W in --neighborhood wraps every axis)\n";
 
 // ---- RuleAggregator (src/common/ruleAggregator.h:24-135) ----
 struct RuleStats {
@@ -302,7 +304,10 @@ int run_search(const Args &a)
     return 0;
 }
 
-int run_step(const Args &a)
+// --mode step, or --mode render --headless: the headless step loop
+// (renderingMode.cpp:166-343).  The render mode's random spawn is
+// cellInit.slang's (no seed salt, float threshold); W in --neighborhood wraps.
+int run_step(const Args &a, bool render)
 {
     const std::string nb = a.str("neighborhood", "F");
     const uint32_t flags = parse_flags(nb, true);
@@ -328,7 +333,8 @@ int run_step(const Args &a)
     spec.params.birthAndMaxCellStateRules = ((uint64_t)((M - 1) & 0xF) << 60) | B;  // guiSystem.cpp:370-388
     spec.params.survivalAndNeighborhoodRules = ((uint64_t)flags << 60) | S;
     spec.spawn[0] = spec.spawn[1] = spec.spawn[2] = (uint32_t)std::min(spawn, grid);
-    spec.spawn_mode = spawn_mode == "random" ? CAVE_SPAWN_RANDOM : CAVE_SPAWN_FILLED;
+    spec.spawn_mode = spawn_mode == "random" ? (render ? CAVE_SPAWN_RENDER_RANDOM : CAVE_SPAWN_RANDOM)
+                                             : CAVE_SPAWN_FILLED;
     spec.density = std::clamp(a.real("spawn-density", 0.5), 0.0, 1.0);
     spec.seed = (uint32_t)a.integer("seed", 0);
     CaveStep *h = nullptr;
@@ -370,7 +376,7 @@ int main(int argc, char **argv)
         }
         const std::string mode = a.str("mode", "");
         if (mode == "search") return run_search(a);
-        if (mode == "step" || (mode == "render" && a.flag("headless"))) return run_step(a);
+        if (mode == "step" || (mode == "render" && a.flag("headless"))) return run_step(a, mode == "render");
         log_error("Unknown mode: " + mode + ". Use search or step (render / video / looking-glass are not part of this build).");
         return 1;
     } catch (const ParseError &e) {

@@ -17,6 +17,7 @@ std::string bitmask_to_rule_string(uint64_t mask);
 uint64_t splitmix64(uint64_t &s);
 double uniform01(uint64_t &s);
 uint32_t density_threshold(double density);
+uint32_t render_density_threshold(double density, bool *all_alive);
 DevInit make_init(const uint32_t dims[3], const uint32_t spawn[3], uint32_t spawn_mode, double density,
                   uint32_t seed);
 

@@ -45,6 +45,18 @@ __device__ inline uint32_t live_mask8(uint32_t w, uint32_t M)
     return m;
 }
 
+// Live bit of cell x of a row (x taken modulo X).
+__device__ inline uint32_t live_bit_wrapped(const uint32_t *row, int x, uint32_t X, uint32_t M)
+{
+    const uint32_t xw = (uint32_t)(((x % (int)X) + (int)X) % (int)X);
+    return ((row[xw >> 3] >> (4u * (xw & 7u))) & 15u) == M ? 1u : 0u;
+}
+
+// WRAP: every axis wraps (rayMarchCompute.slang:80-108 with WRAP_AT_BOUNDARY
+// on one domain); each of the 26 offsets is its own term, so on an axis
+// shorter than 3 cells the same cell -- or the cell itself -- is counted once
+// per offset that reaches it.
+template <bool WRAP>
 __global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                      GridGeom g, DevRule rule, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
@@ -67,9 +79,17 @@ __global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict_
             for (int dz = -1; dz <= 1; dz++)
 #pragma unroll
                 for (int dy = -1; dy <= 1; dy++) {
-                    const int yy = y + dy, zz = z + dz;
+                    int yy = y + dy, zz = z + dz;
                     uint32_t m = 0;
-                    if (yy >= 0 && yy < (int)g.Y && zz >= 0 && zz < (int)g.Z) {
+                    if constexpr (WRAP) {
+                        // bits 7..16: cells 8*xw-1 .. 8*xw+8, each taken modulo X (a
+                        // cell past the row end is the row's first cells)
+                        yy = (yy + (int)g.Y) % (int)g.Y;
+                        zz = (zz + (int)g.Z) % (int)g.Z;
+                        const uint32_t *rowp = in + ((size_t)zz * g.Y + yy) * g.pitch;
+                        for (int j = 0; j < 10; j++)
+                            m |= live_bit_wrapped(rowp, (int)(xw * 8u) - 1 + j, g.X, rule.M) << (7 + j);
+                    } else if (yy >= 0 && yy < (int)g.Y && zz >= 0 && zz < (int)g.Z) {
                         const size_t base = ((size_t)zz * g.Y + yy) * g.pitch;
                         const uint32_t wl = xw > 0 ? in[base + xw - 1] : 0u;
                         const uint32_t wc = in[base + xw];
@@ -264,9 +284,14 @@ uint32_t gather_blocks(const GridGeom &g)
 }
 
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
-                  hipStream_t st, const uint32_t *done)
+                  hipStream_t st, const uint32_t *done, bool wrap)
 {
-    hipLaunchKernelGGL(k_step_gather, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials, done);
+    if (wrap)
+        hipLaunchKernelGGL(k_step_gather<true>, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials,
+                           done);
+    else
+        hipLaunchKernelGGL(k_step_gather<false>, dim3(gather_blocks(g)), dim3(256), 0, st, in, out, g, rule, partials,
+                           done);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

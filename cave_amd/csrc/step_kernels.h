@@ -32,8 +32,9 @@ int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st);
 uint32_t gather_blocks(const GridGeom &g);
 // done (nullable): device flag; when set the launch does nothing (the plain
 // per-config driver ends a simulation on the device without a host round trip)
+// wrap: every axis wraps (the render path's W flag)
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
-                  hipStream_t st, const uint32_t *done = nullptr);
+                  hipStream_t st, const uint32_t *done = nullptr, bool wrap = false);
 
 int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
                         hipStream_t st);

@@ -132,7 +132,11 @@ typedef struct CaveStep CaveStep;
 enum {
     CAVE_SPAWN_RANDOM = 0, /* hash(x,y,z,seed) < floor(density * 2^32) inside the box */
     CAVE_SPAWN_FILLED = 1, /* every box cell live */
-    CAVE_SPAWN_BUFFER = 2  /* caller-supplied cells (reference packed layout) */
+    CAVE_SPAWN_BUFFER = 2, /* caller-supplied cells (reference packed layout) */
+    /* the render path's initial cells (shaders/slang/simulation/cellInit.slang:33-60):
+     * the cell hash without a seed, threshold uint(d * 2^32) computed in float from
+     * the "%f" text of the float density (simulationRenderer.cpp:201) */
+    CAVE_SPAWN_RENDER_RANDOM = 3
 };
 
 enum {

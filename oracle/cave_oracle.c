@@ -89,6 +89,37 @@ uint32_t orc_density_threshold(double density)
     return (density >= 1.0) ? 0xFFFFFFFFu : (uint32_t)(density * 4294967296.0);
 }
 
+uint32_t orc_render_density_threshold(double density, int *all_alive)
+{   /* cellInit.slang:53-60: the density is the float Simulation keeps
+     * (simulation.h:19), passed as the std::to_string text of that float
+     * (simulationRenderer.cpp:201), i.e. "%f"; threshold = uint(d * 2^32) in float */
+    char text[64];
+    snprintf(text, sizeof text, "%f", (double)(float)density);
+    const float d = strtof(text, NULL);
+    *all_alive = d >= 1.0f;
+    if (*all_alive) return 0u;
+    const float t = d * 4294967296.0f;
+    return t <= 0.0f ? 0u : (uint32_t)t;
+}
+
+void orc_init_cells_render(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state,
+                           uint32_t sx, uint32_t sy, uint32_t sz, uint32_t spawn_mode, double density)
+{   /* cellInit.slang:33-60, 105-144: the spawn box of simulationRenderer.cpp:118-128,
+     * the search hash without a seed, the float threshold above */
+    memset(cells, 0, orc_cell_words(X, Y, Z) * sizeof(uint32_t));
+    int all = 0;
+    const uint32_t threshold = orc_render_density_threshold(density, &all);
+    for (uint32_t z = 0; z < Z; z++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t x = 0; x < X; x++) {
+                int in_box = x >= (X - sx) / 2 && x < (X + sx) / 2 && y >= (Y - sy) / 2 && y < (Y + sy) / 2 &&
+                             z >= (Z - sz) / 2 && z < (Z + sz) / 2;
+                if (!in_box) continue;
+                int alive = spawn_mode == 0 ? (all || orc_cell_hash(x, y, z, 0) < threshold) : 1;
+                if (alive) cells[word_index(X, Y, x, y, z)] |= (max_cell_state & 0xFu) << ((x % 8u) * 4u);
+            }
+}
+
 void orc_init_cells(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z,
                     uint32_t max_cell_state, uint32_t sx, uint32_t sy, uint32_t sz,
                     uint32_t spawn_mode, double density, uint32_t seed)
@@ -122,34 +153,45 @@ static const int kEdge[12][3] = {{1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0},
 static const int kCorner[8][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1},
                                   {1, 1, -1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
 
-/* searchSimulationCube.slang:78-94 */
+static int wrap_coord(int c, int n) { return ((c % n) + n) % n; }
+
+/* searchSimulationCube.slang:78-94; with wrap, rayMarchCompute.slang:80-108
+ * (every axis taken modulo its size: WRAP_AT_BOUNDARY, one domain) */
 static int is_alive_neighbor(const uint32_t *in, int cx, int cy, int cz, uint32_t M,
-                             uint32_t X, uint32_t Y, uint32_t Z)
+                             uint32_t X, uint32_t Y, uint32_t Z, int wrap)
 {
-    if (cx < 0 || cx >= (int)X || cy < 0 || cy >= (int)Y || cz < 0 || cz >= (int)Z) return 0;
+    if (wrap) {
+        cx = wrap_coord(cx, (int)X);
+        cy = wrap_coord(cy, (int)Y);
+        cz = wrap_coord(cz, (int)Z);
+    } else if (cx < 0 || cx >= (int)X || cy < 0 || cy >= (int)Y || cz < 0 || cz >= (int)Z) {
+        return 0;
+    }
     return orc_get_cell(in, X, Y, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz) == M;
 }
 
-/* searchSimulationCube.slang:96-115 */
+/* searchSimulationCube.slang:96-115 (rayMarchCompute.slang:115-148 for cubes):
+ * one term per offset, so on a wrapped grid smaller than 3 cells along an
+ * axis the same cell (or the cell itself) is counted once per offset */
 static int count_alive(const uint32_t *in, int x, int y, int z, uint32_t M, uint32_t X,
-                       uint32_t Y, uint32_t Z, uint32_t flags)
+                       uint32_t Y, uint32_t Z, uint32_t flags, int wrap)
 {
     int c = 0;
     if (flags & 1u)
         for (int i = 0; i < 6; i++)
-            c += is_alive_neighbor(in, x + kFace[i][0], y + kFace[i][1], z + kFace[i][2], M, X, Y, Z);
+            c += is_alive_neighbor(in, x + kFace[i][0], y + kFace[i][1], z + kFace[i][2], M, X, Y, Z, wrap);
     if (flags & 2u)
         for (int i = 0; i < 12; i++)
-            c += is_alive_neighbor(in, x + kEdge[i][0], y + kEdge[i][1], z + kEdge[i][2], M, X, Y, Z);
+            c += is_alive_neighbor(in, x + kEdge[i][0], y + kEdge[i][1], z + kEdge[i][2], M, X, Y, Z, wrap);
     if (flags & 4u)
         for (int i = 0; i < 8; i++)
-            c += is_alive_neighbor(in, x + kCorner[i][0], y + kCorner[i][1], z + kCorner[i][2], M, X, Y, Z);
+            c += is_alive_neighbor(in, x + kCorner[i][0], y + kCorner[i][1], z + kCorner[i][2], M, X, Y, Z, wrap);
     return c;
 }
 
 static void step_impl(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z,
                       uint64_t word0, uint64_t word1, const uint64_t *valid_bitmasks,
-                      uint32_t max_rule_bits, orc_tick_stats *st)
+                      uint32_t max_rule_bits, orc_tick_stats *st, int wrap)
 {
     /* searchSimulationCube.slang:139-157 */
     uint32_t M = (uint32_t)(word0 >> 60) + 1u;
@@ -173,7 +215,7 @@ static void step_impl(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y,
         for (uint32_t y = 0; y < Y; y++)
             for (uint32_t x = 0; x < X; x++) {
                 uint32_t cur = orc_get_cell(in, X, Y, x, y, z);
-                int n = count_alive(in, (int)x, (int)y, (int)z, M, X, Y, Z, flags);
+                int n = count_alive(in, (int)x, (int)y, (int)z, M, X, Y, Z, flags, wrap);
                 uint32_t ns;
                 /* searchSimulationCube.slang:178-211 */
                 if (cur == M) {
@@ -204,7 +246,7 @@ void orc_step(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_
               uint32_t max_rule_bits, orc_snapshot *snap)
 {
     orc_tick_stats st;
-    step_impl(in, out, X, Y, Z, word0, word1, valid_bitmasks, max_rule_bits, &st);
+    step_impl(in, out, X, Y, Z, word0, word1, valid_bitmasks, max_rule_bits, &st, 0);
     *snap = st.snap;
 }
 
@@ -212,7 +254,14 @@ void orc_step_stats(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, u
                     uint64_t word0, uint64_t word1, const uint64_t *valid_bitmasks,
                     uint32_t max_rule_bits, orc_tick_stats *stats)
 {
-    step_impl(in, out, X, Y, Z, word0, word1, valid_bitmasks, max_rule_bits, stats);
+    step_impl(in, out, X, Y, Z, word0, word1, valid_bitmasks, max_rule_bits, stats, 0);
+}
+
+void orc_step_render(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z,
+                     uint64_t word0, uint64_t word1, orc_tick_stats *stats)
+{   /* rayMarchCompute.slang:150-178, 222-281: raw bitmask rules; the W flag
+     * (bit 3 of the neighbourhood nibble, simulationRenderer.cpp:595) wraps */
+    step_impl(in, out, X, Y, Z, word0, word1, NULL, 0, stats, (int)((word1 >> 60) & 8u) != 0);
 }
 
 /* ------------------------------------------------------------------ */
@@ -463,7 +512,7 @@ static uint32_t run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_s
     while (1) {
         orc_tick_stats st;
         step_impl(a, b, X, Y, Z, perm.birthAndMaxCellStateRules, perm.survivalAndNeighborhoodRules,
-                  NULL, 0, &st);
+                  NULL, 0, &st, 0);
         if (per_tick && t < max_ticks) per_tick[t] = st;
         t++;
         /* the copy-based ping-pong of searchSystem.cpp:599-616 */

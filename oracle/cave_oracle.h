@@ -99,6 +99,15 @@ void orc_init_cells(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z,
 /* ---- step (shaders/slang/search/searchSimulationCube.slang:78-247) ----
  * word0/word1: rule words.  When max_rule_bits > 0 the low 60 bits are
  * 1-based indices into valid_bitmasks, otherwise raw bitmasks. */
+/* the headless render-path step (rayMarchCompute.slang): wraps every axis
+ * when the W flag (word1 bit 63) is set; rules are raw bitmasks */
+void orc_step_render(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z,
+                     uint64_t word0, uint64_t word1, orc_tick_stats *stats);
+/* render-path initial cells (cellInit.slang): unsalted hash, float threshold */
+uint32_t orc_render_density_threshold(double density, int *all_alive);
+void orc_init_cells_render(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state,
+                           uint32_t sx, uint32_t sy, uint32_t sz, uint32_t spawn_mode, double density);
+
 void orc_step(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z,
               uint64_t word0, uint64_t word1, const uint64_t *valid_bitmasks,
               uint32_t max_rule_bits, orc_snapshot *snap);

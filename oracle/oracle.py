@@ -103,6 +103,12 @@ def lib():
         L.orc_run_search.restype = u64
         L.orc_run_search.argtypes = [u32, u32, u32, C.c_int, C.c_int, u32, u32, u32, C.c_double, u32,
                                      C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_step_render.restype = None
+        L.orc_step_render.argtypes = [p32, p32, u32, u32, u32, u64, u64, C.POINTER(TickStats)]
+        L.orc_render_density_threshold.restype = u32
+        L.orc_render_density_threshold.argtypes = [C.c_double, C.POINTER(C.c_int)]
+        L.orc_init_cells_render.restype = None
+        L.orc_init_cells_render.argtypes = [p32, u32, u32, u32, u32, u32, u32, u32, u32, C.c_double]
         _lib = L
     return _lib
 
@@ -120,6 +126,28 @@ def init_cells(X, Y, Z, M, spawn, spawn_mode, density, seed) -> np.ndarray:
     a = np.zeros(cell_words(X, Y, Z), dtype=np.uint32)
     sx, sy, sz = spawn
     lib().orc_init_cells(_p32(a), X, Y, Z, M, sx, sy, sz, spawn_mode, float(density), seed)
+    return a
+
+
+def step_render(cells, X, Y, Z, word0, word1):
+    """The headless render-path step (wraps every axis when word1 has the W flag)."""
+    out = np.zeros_like(cells)
+    st = TickStats()
+    lib().orc_step_render(_p32(cells), _p32(out), X, Y, Z, word0, word1, C.byref(st))
+    return out, st
+
+
+def render_density_threshold(density):
+    """(threshold, all_alive) of the render path's random spawn."""
+    a = C.c_int(0)
+    t = lib().orc_render_density_threshold(float(density), C.byref(a))
+    return int(t), bool(a.value)
+
+
+def init_cells_render(X, Y, Z, M, spawn, spawn_mode, density) -> np.ndarray:
+    a = np.zeros(cell_words(X, Y, Z), dtype=np.uint32)
+    sx, sy, sz = spawn
+    lib().orc_init_cells_render(_p32(a), X, Y, Z, M, sx, sy, sz, spawn_mode, float(density))
     return a
 
 

@@ -40,8 +40,15 @@ def offsets(flags: int):
             yield d
 
 
-def neighbor_counts(live: np.ndarray, flags: int) -> np.ndarray:
+def neighbor_counts(live: np.ndarray, flags: int, wrap: bool = False) -> np.ndarray:
     Z, Y, X = live.shape
+    if wrap:
+        # toroidal: one rolled copy per offset, so a short axis counts the same
+        # cell (or the cell itself) once per offset that reaches it
+        n = np.zeros((Z, Y, X), dtype=np.int32)
+        for dx, dy, dz in offsets(flags):
+            n += np.roll(live, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
+        return n
     p = np.zeros((Z + 2, Y + 2, X + 2), dtype=np.int32)
     p[1:-1, 1:-1, 1:-1] = live
     n = np.zeros((Z, Y, X), dtype=np.int32)
@@ -50,10 +57,10 @@ def neighbor_counts(live: np.ndarray, flags: int) -> np.ndarray:
     return n
 
 
-def step(cells: np.ndarray, M: int, flags: int, birth: int, survival: int):
+def step(cells: np.ndarray, M: int, flags: int, birth: int, survival: int, wrap: bool = False):
     """Returns (new cells, (dead, dying, born, surviving))."""
     live = (cells == M).astype(np.int32)
-    n = neighbor_counts(live, flags)
+    n = neighbor_counts(live, flags, wrap)
     bit = lambda mask: np.where(n > 0, (np.uint64(mask) >> (np.maximum(n, 1) - 1).astype(np.uint64)) & np.uint64(1), 0).astype(bool)
     surv = (cells == M) & bit(survival)
     born = (cells == 0) & bit(birth)

@@ -83,10 +83,51 @@ def test_random_grids_state_by_state(kernel, X, Y, Z, M, flags):
             assert np.array_equal(h.cells(), ref), f"tick {t}"
 
 
-def test_wrap_flag_rejected():
+WRAP_CASES = [
+    # (X, Y, Z, M, flags): tiny tori count the same cell once per offset that reaches it
+    (1, 1, 1, 1, 7), (2, 3, 1, 2, 3), (3, 2, 2, 4, 5), (7, 5, 3, 1, 7), (9, 8, 10, 3, 1),
+    (33, 17, 9, 2, 3), (64, 3, 5, 15, 6), (100, 31, 7, 5, 7), (17, 1, 40, 1, 2),
+]
+
+
+@pytest.mark.parametrize("X,Y,Z,M,flags", WRAP_CASES)
+def test_wrap_grids_state_by_state(X, Y, Z, M, flags):
+    """The W flag wraps every axis (the render path's step,
+    rayMarchCompute.slang:80-108): GPU vs oracle, state and counters per tick."""
+    import np_model as NP
+
+    rng = np.random.default_rng(X * 131 + Y * 17 + Z + M * 999 + flags)
+    n = O.lib().orc_neighbor_count(flags)
+    birth = int(rng.integers(1, 1 << n))
+    surv = int(rng.integers(1, 1 << n))
+    dense = rng.integers(0, M + 1, size=(Z, Y, X)).astype(np.uint8)
+    packed = NP.pack(dense)
+    w0 = ((M - 1) & 0xF) << 60 | birth
+    w1 = (flags | cv.WRAP) << 60 | surv
+    print(f"wrap case {X}x{Y}x{Z} M={M} flags={flags}", flush=True)
+    with cv.Step((X, Y, Z), M, flags | cv.WRAP, birth, surv, initial_cells=packed) as h:
+        assert h.kernel == cv.KERNEL_GATHER
+        ref = packed
+        for t in range(6):
+            ref, st_ref = O.step_render(ref, X, Y, Z, w0, w1)
+            (st,), _ = h.run(1)
+            assert st.as_tuple() == st_ref.as_tuple(), f"tick {t}"
+            assert np.array_equal(h.cells(), ref), f"tick {t}"
+
+
+def test_wrap_needs_the_gather_kernel():
     with pytest.raises(cv.CaveError) as e:
-        cv.Step((8, 8, 8), 1, cv.FACES | cv.WRAP, 1, 1)
+        cv.Step((8, 8, 8), 1, cv.FACES | cv.WRAP, 1, 1, kernel=cv.KERNEL_STREAM)
     assert e.value.code == cv.CAVE_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("density", [0.35, 0.2, 1.0 / 3.0, 1.0])
+def test_render_random_spawn(density):
+    """cellInit.slang's initial cells: unsalted hash, float threshold."""
+    X, Y, Z, M, spawn = 70, 33, 21, 6, (40, 20, 15)
+    with cv.Step((X, Y, Z), M, cv.FACES | cv.EDGES, 1, 1, spawn=spawn, spawn_mode=cv.SPAWN_RENDER_RANDOM,
+                 density=density, seed=12345) as h:
+        assert np.array_equal(h.cells(), O.init_cells_render(X, Y, Z, M, spawn, 0, density))
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
@@ -130,3 +171,35 @@ def test_garbage_row_padding_is_ignored(kernel):
             (st,), _ = h.run(1)
             assert st.as_tuple() == st_ref.as_tuple(), f"tick {t}"
             assert np.array_equal(h.cells(), ref), f"tick {t}"
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
+def test_graph_replays_match_eager_and_oracle(kernel):
+    """A run of ticks is a captured HIP graph replayed per call: replays of
+    both buffer parities, interleaved with eager work and host syncs, equal
+    one-by-one launches and the oracle after every call."""
+    import os as _os
+
+    X, Y, Z, M, flags = 70, 23, 19, 4, 7
+    rng = np.random.default_rng(5)
+    n = O.lib().orc_neighbor_count(flags)
+    birth, surv = int(rng.integers(1, 1 << n)), int(rng.integers(1, 1 << n))
+    import np_model as NP
+    packed = NP.pack(rng.integers(0, M + 1, size=(Z, Y, X)).astype(np.uint8))
+    w0 = ((M - 1) & 0xF) << 60 | birth
+    w1 = flags << 60 | surv
+    eager = cv.Step((X, Y, Z), M, flags, birth, surv, initial_cells=packed, kernel=kernel)
+    _os.environ["CAVE_STEP_GRAPHS"] = "1"
+    try:
+        graphed = cv.Step((X, Y, Z), M, flags, birth, surv, initial_cells=packed, kernel=kernel)
+    finally:
+        del _os.environ["CAVE_STEP_GRAPHS"]
+    ref = packed
+    with graphed as h, eager:
+        for ticks in (5, 5, 3, 5, 1, 5, 3, 3):
+            rows, _ = h.run(ticks)
+            erows, _ = eager.run(ticks)
+            for t in range(ticks):
+                ref, st = O.step(ref, X, Y, Z, w0, w1)
+                assert rows[t].as_tuple() == st.as_tuple() == erows[t].as_tuple()
+            assert np.array_equal(h.cells(), ref) and np.array_equal(eager.cells(), ref)

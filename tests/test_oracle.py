@@ -274,3 +274,72 @@ def test_run_rule_evaluates_all_zero_first_word():
     # survival of every count keeps the live cells with a live neighbour
     t, info, _, _ = O.run_rule(9, 9, 9, 1, 1, 0, 0x3F, 40, init)
     assert info.numberOfSimulationTicksSurvived == t and t >= 1
+
+
+# ---- the headless render-path step (rayMarchCompute.slang) and init (cellInit.slang) ----
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 1), (3, 2, 2), (5, 4, 3), (9, 7, 6), (17, 5, 4)])
+@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5, 7])
+def test_render_step_wraps_like_a_torus(dims, flags):
+    X, Y, Z = dims
+    rng = np.random.default_rng(X * 100 + Y * 10 + Z + flags)
+    for M in (1, 3, 15):
+        cells = rng.integers(0, M + 1, size=(Z, Y, X)).astype(np.uint8)
+        birth, surv = int(rng.integers(1, 1 << 26)), int(rng.integers(1, 1 << 26))
+        w0 = ((M - 1) & 0xF) << 60 | birth
+        w1 = (flags | 8) << 60 | surv
+        out, st = O.step_render(NP.pack(cells), X, Y, Z, w0, w1)
+        want, counts = NP.step(cells, M, flags, birth, surv, wrap=True)
+        assert np.array_equal(NP.unpack(out, X, Y, Z), want)
+        assert st.as_tuple()[:4] == counts
+        # without W the render step is the search step
+        w1n = flags << 60 | surv
+        a, _ = O.step_render(NP.pack(cells), X, Y, Z, w0, w1n)
+        b, _ = O.step(NP.pack(cells), X, Y, Z, w0, w1n)
+        assert np.array_equal(a, b)
+
+
+def test_render_step_self_counts_on_one_cell_torus():
+    # a 1^3 torus: all 26 offsets reach the cell itself
+    cells = np.array([[[1]]], dtype=np.uint8)
+    for flags, n in ((1, 6), (2, 12), (4, 8), (7, 26)):
+        w0 = 0 << 60 | (1 << (n - 1))  # M = 1, born on n
+        w1 = (flags | 8) << 60 | (1 << (n - 1))  # survives on exactly n
+        out, st = O.step_render(NP.pack(cells), 1, 1, 1, w0, w1)
+        assert st.snap.numberOfSurvivingCells == 1
+
+
+@pytest.mark.parametrize("p,thr", [(0.35, 1503238528), (0.5, 2147483648), (0.25, 1073741824),
+                                   (0.2, 858993472), (0.3, 1288490240), (0.0, 0)])
+def test_render_density_threshold(p, thr):
+    # uint(float(p) * 2^32) after the "%f" round trip; the search path's double
+    # floor(p * 2^32) differs for p = 0.35 (1503238553), 0.2 (858993459), 0.3
+    t, all_alive = O.render_density_threshold(p)
+    assert (t, all_alive) == (thr, False)
+    f = np.float32(float("%f" % np.float32(p)))
+    assert int(np.float32(f) * np.float32(2.0**32)) == thr
+
+
+def test_render_density_threshold_round_trip_and_full():
+    # 1/3 becomes "0.333333" first
+    t, _ = O.render_density_threshold(1.0 / 3.0)
+    assert t == int(np.float32("0.333333") * np.float32(2.0**32))
+    assert O.render_density_threshold(1.0) == (0, True)
+    assert O.render_density_threshold(0.9999999)[1]  # "%f" rounds it to 1.000000
+
+
+def test_render_init_is_unsalted_hash_with_float_threshold():
+    X, Y, Z, M, spawn = 23, 19, 17, 5, (12, 9, 11)
+    for p in (0.35, 0.5):
+        got = NP.unpack(O.init_cells_render(X, Y, Z, M, spawn, 0, p), X, Y, Z)
+        thr, _ = O.render_density_threshold(p)
+        want = np.zeros((Z, Y, X), np.uint8)
+        for z in range((Z - spawn[2]) // 2, (Z + spawn[2]) // 2):
+            for y in range((Y - spawn[1]) // 2, (Y + spawn[1]) // 2):
+                for x in range((X - spawn[0]) // 2, (X + spawn[0]) // 2):
+                    if O.lib().orc_cell_hash(x, y, z, 0) < thr:
+                        want[z, y, x] = M
+        assert np.array_equal(got, want)
+    # where the two thresholds agree, the render init is the search init with seed 0
+    a = O.init_cells_render(X, Y, Z, M, spawn, 0, 0.5)
+    b = O.init_cells(X, Y, Z, M, spawn, 0, 0.5, 0)
+    assert np.array_equal(a, b)
