@@ -148,6 +148,9 @@ _SIGS = {
     "cave_step_destroy": (C.c_int, [_vp]),
     "cave_batch_run": (C.c_int, [C.POINTER(SimSpec), _sz, C.POINTER(SimResult), C.POINTER(BatchOptions)]),
     "cave_batch_tuning_report": (C.c_int, [C.c_char_p, _sz]),
+    "cave_render_bytes": (_sz, [_u32, _u32, _u32, _u32]),
+    "cave_render_to_packed": (C.c_int, [_u32, _vp, _u32, _u32, _u32, C.POINTER(_u32)]),
+    "cave_packed_to_render": (C.c_int, [_u32, C.POINTER(_u32), _u32, _u32, _u32, _vp]),
     "cave_search_create": (C.c_int, [C.POINTER(SearchSpec), C.POINTER(_vp)]),
     "cave_search_step": (C.c_int, [_vp, _u64]),
     "cave_search_is_running": (C.c_int, [_vp]),
@@ -379,6 +382,26 @@ def batch_run_np(specs: np.ndarray, engine=ENGINE_AUTO, block_threads=0, max_par
     if timing:
         return res, opt.device_ms, tm, opt.block_threads
     return res, opt.device_ms
+
+
+RENDER_IMAGE, RENDER_TILED = 0, 1
+
+
+def render_to_packed(layout: int, render: np.ndarray, X: int, Y: int, Z: int) -> np.ndarray:
+    """Render-path storage (image bytes or tiled u32 buffer) -> packed search layout."""
+    src = np.ascontiguousarray(render)
+    out = np.zeros(cell_words(X, Y, Z), dtype=np.uint32)
+    check(lib().cave_render_to_packed(layout, src.ctypes.data, X, Y, Z, _p32(out)))
+    return out
+
+
+def packed_to_render(layout: int, packed: np.ndarray, X: int, Y: int, Z: int) -> np.ndarray:
+    """Packed search layout -> render-path storage (uint8 image or uint32 tiled buffer)."""
+    n = int(lib().cave_render_bytes(layout, X, Y, Z))
+    out = np.zeros(n if layout == RENDER_IMAGE else n // 4, dtype=np.uint8 if layout == RENDER_IMAGE else np.uint32)
+    src = np.ascontiguousarray(packed, dtype=np.uint32)
+    check(lib().cave_packed_to_render(layout, _p32(src), X, Y, Z, out.ctypes.data))
+    return out
 
 
 def tuning_report():

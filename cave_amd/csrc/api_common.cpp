@@ -213,3 +213,81 @@ uint64_t cave_random_rule(uint64_t *state, uint32_t n_counts, double p)
 }
 
 }  // extern "C"
+
+// ---- render-path storage (cellStateUtils.slang:18-46) ----
+namespace {
+
+uint32_t spread3(uint32_t v)
+{   // bits b0 b1 b2 -> positions 0, 3, 6
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+
+struct RenderGeom {
+    uint32_t iw, nbx, nby, nbz;
+    RenderGeom(uint32_t X, uint32_t Y, uint32_t Z)
+        : iw((X + 1u) / 2u), nbx((iw + 7u) / 8u), nby((Y + 7u) / 8u), nbz((Z + 7u) / 8u) {}
+    size_t tiled_index(uint32_t tx, uint32_t y, uint32_t z) const
+    {
+        const size_t block = ((size_t)(z >> 3) * nby + (y >> 3)) * nbx + (tx >> 3);
+        return block * 512u + (spread3(tx & 7u) | (spread3(y & 7u) << 1) | (spread3(z & 7u) << 2));
+    }
+};
+
+}  // namespace
+
+extern "C" size_t cave_render_bytes(uint32_t layout, uint32_t X, uint32_t Y, uint32_t Z)
+{
+    const RenderGeom r(X, Y, Z);
+    if (layout == CAVE_RENDER_IMAGE) return (size_t)r.iw * Y * Z;
+    if (layout == CAVE_RENDER_TILED) return (size_t)r.nbx * r.nby * r.nbz * 512u * 4u;
+    return 0;
+}
+
+extern "C" int cave_render_to_packed(uint32_t layout, const void *render, uint32_t X, uint32_t Y, uint32_t Z,
+                                     uint32_t *packed)
+{
+    if (!render || !packed || !X || !Y || !Z) return fail(CAVE_E_INVALID, "null argument or empty grid");
+    if (layout > CAVE_RENDER_TILED) return fail(CAVE_E_INVALID, "bad render layout");
+    const RenderGeom r(X, Y, Z);
+    const uint32_t upr = (X + 7u) / 8u;
+    std::memset(packed, 0, sizeof(uint32_t) * (size_t)upr * Y * Z);
+    const uint8_t *img = static_cast<const uint8_t *>(render);
+    const uint32_t *buf = static_cast<const uint32_t *>(render);
+    for (uint32_t z = 0; z < Z; z++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t tx = 0; tx < r.iw; tx++) {
+                const uint32_t byte = layout == CAVE_RENDER_IMAGE ? img[((size_t)z * Y + y) * r.iw + tx]
+                                                                  : buf[r.tiled_index(tx, y, z)] & 0xFFu;
+                for (uint32_t k = 0; k < 2; k++) {
+                    const uint32_t x = 2u * tx + k;
+                    if (x >= X) break;
+                    packed[(size_t)(z * Y + y) * upr + x / 8u] |= ((byte >> (4u * k)) & 15u) << (4u * (x % 8u));
+                }
+            }
+    return CAVE_OK;
+}
+
+extern "C" int cave_packed_to_render(uint32_t layout, const uint32_t *packed, uint32_t X, uint32_t Y, uint32_t Z,
+                                     void *render)
+{
+    if (!render || !packed || !X || !Y || !Z) return fail(CAVE_E_INVALID, "null argument or empty grid");
+    if (layout > CAVE_RENDER_TILED) return fail(CAVE_E_INVALID, "bad render layout");
+    const RenderGeom r(X, Y, Z);
+    const uint32_t upr = (X + 7u) / 8u;
+    std::memset(render, 0, cave_render_bytes(layout, X, Y, Z));
+    uint8_t *img = static_cast<uint8_t *>(render);
+    uint32_t *buf = static_cast<uint32_t *>(render);
+    for (uint32_t z = 0; z < Z; z++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t tx = 0; tx < r.iw; tx++) {
+                uint32_t byte = 0;
+                for (uint32_t k = 0; k < 2; k++) {
+                    const uint32_t x = 2u * tx + k;
+                    if (x >= X) break;
+                    byte |= ((packed[(size_t)(z * Y + y) * upr + x / 8u] >> (4u * (x % 8u))) & 15u) << (4u * k);
+                }
+                if (layout == CAVE_RENDER_IMAGE) img[((size_t)z * Y + y) * r.iw + tx] = (uint8_t)byte;
+                else buf[r.tiled_index(tx, y, z)] = byte;
+            }
+    return CAVE_OK;
+}

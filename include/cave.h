@@ -163,6 +163,21 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out);
  * device_ms (nullable) receives the device time of the tick loop, measured
  * with HIP events on the handle's stream. */
 int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *device_ms);
+/* ---- the render path's cell storage (shaders/slang/common/cellStateUtils.slang:1-70) ----
+ * Two cells per byte along x (low nibble = even x).  CAVE_RENDER_IMAGE: the
+ * r8ui 3D image, ceil(X/2) x Y x Z bytes, x fastest.  CAVE_RENDER_TILED: the
+ * buffer form, one packed byte per u32 element, 8x8x8 texel blocks (block
+ * index z-major, then y, then x) with Morton order inside a block
+ * (cellStateUtils.slang:18-46).  Host-side conversions to and from the
+ * packed search layout (cave_cell_words words). */
+enum { CAVE_RENDER_IMAGE = 0, CAVE_RENDER_TILED = 1 };
+/* Bytes of a render-layout buffer for a grid. */
+size_t cave_render_bytes(uint32_t layout, uint32_t X, uint32_t Y, uint32_t Z);
+int cave_render_to_packed(uint32_t layout, const void *render, uint32_t X, uint32_t Y, uint32_t Z,
+                          uint32_t *packed);
+int cave_packed_to_render(uint32_t layout, const uint32_t *packed, uint32_t X, uint32_t Y, uint32_t Z,
+                          void *render);
+
 /* Copies the current cells out in the reference packed layout. */
 int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words);
 int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words);
