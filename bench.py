@@ -253,7 +253,7 @@ def run_uneven_bench(n_configs=2048, cpu=True):
     return out
 
 
-def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True):
+def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0):
     """Online serving: Poisson arrivals at rate_hz for duration_s, each request
     a grid of edge 32..192 (log-uniform), F/FE/FEV, C in 2..16, bits p = 0.12,
     20..500 ticks, admitted against a 200 GB HBM budget into the resident
@@ -272,7 +272,7 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True):
     sample = np.arange(0, n, 100)
     specs["sim_flags"][sample] |= cv.SIM_FINAL_HASH
     specs = np.ascontiguousarray(specs)
-    srv = cv.Server(hbm_budget_bytes=200 << 30)
+    srv = cv.Server(hbm_budget_bytes=200 << 30, split_target_us=split_us)
     for i in range(8):  # warm-up requests (code objects, first arena chunk)
         srv.submit_row(specs, i, 1 << 40 | i)
     srv.drain()
@@ -388,6 +388,10 @@ def main():
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-configs", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--plain-configs", type=int, default=64)
+    ap.add_argument("--serve-split-us", type=int, default=0,
+                    help="split target of the server (0 = its default)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -442,12 +446,14 @@ def main():
         out["cpu_baseline"] = cb
     if not a.no_search:
         log(f"rule search: {a.search_configs} configs x 32 seeds at 49^3")
-        out["search"] = run_search_bench(a.search_configs, cpu=not a.no_cpu)
-        log("rule search, uneven sizes: 2048 configs")
-        out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
+        out["search"] = run_search_bench(a.search_configs, plain_configs=a.plain_configs, cpu=not a.no_cpu)
+        if not a.no_uneven:
+            log("rule search, uneven sizes: 2048 configs")
+            out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
     if not a.no_serve:
         log(f"serving: Poisson {a.serve_rate:.0f} req/s for {a.serve_seconds:.0f} s")
-        out["serving"] = run_serving_bench(a.serve_rate, a.serve_seconds, cpu=not a.no_cpu)
+        out["serving"] = run_serving_bench(a.serve_rate, a.serve_seconds, cpu=not a.no_cpu,
+                                           split_us=a.serve_split_us)
     log("done")
     print(json.dumps(out), flush=True)
 
