@@ -253,7 +253,7 @@ def run_uneven_bench(n_configs=2048, cpu=True):
     return out
 
 
-def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0):
+def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max_parts=0):
     """Online serving: Poisson arrivals at rate_hz for duration_s, each request
     a grid of edge 32..192 (log-uniform), F/FE/FEV, C in 2..16, bits p = 0.12,
     20..500 ticks, admitted against a 200 GB HBM budget into the resident
@@ -272,7 +272,7 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0):
     sample = np.arange(0, n, 100)
     specs["sim_flags"][sample] |= cv.SIM_FINAL_HASH
     specs = np.ascontiguousarray(specs)
-    srv = cv.Server(hbm_budget_bytes=200 << 30, split_target_us=split_us)
+    srv = cv.Server(hbm_budget_bytes=200 << 30, split_target_us=split_us, max_parts=max_parts)
     for i in range(8):  # warm-up requests (code objects, first arena chunk)
         srv.submit_row(specs, i, 1 << 40 | i)
     srv.drain()
@@ -392,6 +392,8 @@ def main():
     ap.add_argument("--plain-configs", type=int, default=64)
     ap.add_argument("--serve-split-us", type=int, default=0,
                     help="split target of the server (0 = its default)")
+    ap.add_argument("--serve-max-parts", type=int, default=0,
+                    help="most blocks per served request (0 = the server's default)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -453,7 +455,7 @@ def main():
     if not a.no_serve:
         log(f"serving: Poisson {a.serve_rate:.0f} req/s for {a.serve_seconds:.0f} s")
         out["serving"] = run_serving_bench(a.serve_rate, a.serve_seconds, cpu=not a.no_cpu,
-                                           split_us=a.serve_split_us)
+                                           split_us=a.serve_split_us, max_parts=a.serve_max_parts)
     log("done")
     print(json.dumps(out), flush=True)
 

@@ -116,7 +116,7 @@ class SearchProgress(C.Structure):
 
 class ServeOptions(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("split_target_us", C.c_uint32),
-                ("idle_exit_us", C.c_uint32), ("reserved", C.c_uint32)]
+                ("idle_exit_us", C.c_uint32), ("max_parts", C.c_uint32)]
 
 
 class ServeCompletion(C.Structure):
@@ -521,8 +521,9 @@ class Server:
     arrive (cave_serve_*).  submit() queues and admits; poll() returns the
     finished ones as ServeCompletion records (copies)."""
 
-    def __init__(self, hbm_budget_bytes=0, max_inflight=0, split_target_us=0, idle_exit_us=0):
+    def __init__(self, hbm_budget_bytes=0, max_inflight=0, split_target_us=0, idle_exit_us=0, max_parts=0):
         o = ServeOptions()
+        o.max_parts = max_parts
         o.hbm_budget_bytes = hbm_budget_bytes
         o.max_inflight = max_inflight
         o.split_target_us = split_target_us

@@ -43,9 +43,9 @@ constexpr uint32_t kSyncPool = 1u << 17;
 constexpr uint32_t kThreads = 1024;
 constexpr uint64_t kDefaultBudget = 200ull << 30;
 constexpr uint32_t kDefaultInflight = 16384;
-constexpr uint32_t kDefaultSplitUs = 4000;
+constexpr uint32_t kDefaultSplitUs = 1000;
 constexpr uint32_t kDefaultIdleUs = 50000;
-constexpr uint32_t kMaxParts = 16;
+constexpr uint32_t kDefaultMaxParts = 64;
 constexpr double kFallbackBlockRate = 1e10;  // cell-updates/s of one block when a bucket was never measured
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -236,7 +236,7 @@ void plan_request(CaveServer *s, Pending &p)
     uint32_t parts = 1;
     if (!in_lds && est_us > s->opt.split_target_us) {
         parts = (uint32_t)std::ceil(est_us / s->opt.split_target_us);
-        parts = std::min({parts, kMaxParts, std::max(1u, j.Z / 4u), std::max(1u, s->resident / 4u)});
+        parts = std::min({parts, s->opt.max_parts, std::max(1u, j.Z / 4u), std::max(1u, s->resident / 4u)});
     }
     j.parts = parts;
     j.zchunks = part_zchunks(j.X, j.Y, j.Z, kThreads * ipt, parts);
@@ -407,6 +407,7 @@ int cave_serve_create(const CaveServeOptions *opt, CaveServer **out)
     if (!s->opt.max_inflight) s->opt.max_inflight = kDefaultInflight;
     if (!s->opt.split_target_us) s->opt.split_target_us = kDefaultSplitUs;
     if (!s->opt.idle_exit_us) s->opt.idle_exit_us = kDefaultIdleUs;
+    if (!s->opt.max_parts) s->opt.max_parts = kDefaultMaxParts;
     s->t0 = std::chrono::steady_clock::now();
     int rc = CAVE_OK;
     auto bail = [&](int code) {

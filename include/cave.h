@@ -306,9 +306,10 @@ typedef struct CaveServer CaveServer;
 typedef struct CaveServeOptions {
     uint64_t hbm_budget_bytes; /* admission budget for resident cell state + final-cell staging (0 = 200 GB) */
     uint32_t max_inflight;     /* requests admitted and not yet finished (0 = 16384) */
-    uint32_t split_target_us;  /* a request whose one-block run is estimated longer is split (0 = 4000) */
+    uint32_t split_target_us;  /* a request whose one-block run is estimated longer is split (0 = 1000) */
     uint32_t idle_exit_us;     /* the resident kernel closes after this long without a claim (0 = 50000) */
-    uint32_t reserved;
+    uint32_t max_parts;        /* most blocks one request is split over (0 = 64; never above a quarter
+                                  of the resident blocks or Z/4) */
 } CaveServeOptions;
 
 typedef struct CaveServeCompletion {
