@@ -21,9 +21,10 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cave.h")
 CAVE_OK, CAVE_E_INVALID, CAVE_E_UNSUPPORTED, CAVE_E_NOMEM, CAVE_E_DEVICE, CAVE_E_BUDGET, CAVE_E_STATE = range(7)
 FACES, EDGES, CORNERS, WRAP = 1, 2, 4, 8
 SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER, SPAWN_RENDER_RANDOM = 0, 1, 2, 3
-KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE, KERNEL_ERD = 0, 1, 2, 3, 4
 ENGINE_AUTO, ENGINE_PERSISTENT, ENGINE_PLAIN = 0, 1, 2
-SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH = 1, 2, 4
+SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH, SIM_SHAPE_ERD = 1, 2, 4, 8
+SHAPE_CUBE, SHAPE_ERD = 0, 1
 
 
 class CaveError(RuntimeError):
@@ -149,6 +150,7 @@ _SIGS = {
     "cave_batch_run": (C.c_int, [C.POINTER(SimSpec), _sz, C.POINTER(SimResult), C.POINTER(BatchOptions)]),
     "cave_batch_tuning_report": (C.c_int, [C.c_char_p, _sz]),
     "cave_render_bytes": (_sz, [_u32, _u32, _u32, _u32]),
+    "cave_erd_cell_words": (_sz, [_u32, _u32, _u32]),
     "cave_render_to_packed": (C.c_int, [_u32, _vp, _u32, _u32, _u32, C.POINTER(_u32)]),
     "cave_packed_to_render": (C.c_int, [_u32, C.POINTER(_u32), _u32, _u32, _u32, _vp]),
     "cave_search_create": (C.c_int, [C.POINTER(SearchSpec), C.POINTER(_vp)]),
@@ -211,6 +213,10 @@ def enumerate_valid_bitmasks(n: int, k: int) -> list[int]:
     return [int(out[i]) for i in range(cnt)]
 
 
+def erd_cell_words(X: int, Y: int, Z: int) -> int:
+    return int(lib().cave_erd_cell_words(X, Y, Z))
+
+
 def cell_words(X: int, Y: int, Z: int) -> int:
     return int(lib().cave_cell_words(X, Y, Z))
 
@@ -254,10 +260,11 @@ class Step:
     """One resident grid advanced tick by tick on the GPU (headless step path)."""
 
     def __init__(self, dims, M, flags, birth, survival, spawn=None, spawn_mode=SPAWN_RANDOM,
-                 density=0.5, seed=0, initial_cells=None, kernel=KERNEL_AUTO):
+                 density=0.5, seed=0, initial_cells=None, kernel=KERNEL_AUTO, shape=SHAPE_CUBE):
         spec = StepSpec()
         spec.dims[:] = list(dims)
-        spec.params = encode_params(M, flags, birth, survival)
+        spec.params = encode_params(M, flags, birth, survival, shape)
+        self.shape = shape
         spec.spawn[:] = list(spawn if spawn is not None else dims)
         spec.density = float(density)
         spec.seed = seed
@@ -288,8 +295,10 @@ class Step:
         return ([arr[i] for i in range(ticks)] if stats else None), ms.value
 
     def cells(self) -> np.ndarray:
+        """Current cells: the packed search layout, or the linear layout for ERD."""
         X, Y, Z = self.dims
-        out = np.zeros(cell_words(X, Y, Z), dtype=np.uint32)
+        n = erd_cell_words(X, Y, Z) if self.shape == SHAPE_ERD else cell_words(X, Y, Z)
+        out = np.zeros(n, dtype=np.uint32)
         check(lib().cave_step_download(self._h, _p32(out), out.size))
         return out
 
@@ -446,12 +455,12 @@ class Search:
 
     def __init__(self, grid, flags, max_rule_bits, min_cs, max_cs, ticks, spawn=11, spawn_mode=SPAWN_FILLED,
                  density_index=7, seed_index=0, partition_index=0, partition_count=1, output_folder=None,
-                 file_prefix="", sim_flags=0):
+                 file_prefix="", sim_flags=0, shape=SHAPE_CUBE):
         g = grid if isinstance(grid, (tuple, list)) else (grid, grid, grid)
         sp = spawn if isinstance(spawn, (tuple, list)) else (spawn, spawn, spawn)
         s = SearchSpec()
         c = s.chunk
-        c.shapeId = 0
+        c.shapeId = shape
         c.gridDimensionX, c.gridDimensionY, c.gridDimensionZ = g
         c.spawnAreaDimensionX, c.spawnAreaDimensionY, c.spawnAreaDimensionZ = [min(a, b) for a, b in zip(sp, g)]
         c.spawnRegionShapeId = spawn_mode

@@ -40,6 +40,7 @@ LIB_SOURCES = [
     "batch.hip",
     "step_gather.hip",
     "step_stream.hip",
+    "step_erd.hip",
 ]
 CLI_SOURCES: list[str] = ["cli.cpp"]
 

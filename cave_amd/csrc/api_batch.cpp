@@ -54,8 +54,10 @@ int validate_sim(const CaveSimSpec &s)
     if ((uint64_t)s.dims[0] * s.dims[1] * s.dims[2] > 0xFFFFFFFFull)
         return fail(CAVE_E_INVALID, "grid larger than 2^32 cells");
     if (s.max_cell_state < 1 || s.max_cell_state > 16) return fail(CAVE_E_INVALID, "max_cell_state must be 1..16");
-    if (s.neighborhood_flags & CAVE_WRAP) return fail(CAVE_E_UNSUPPORTED, "wrap (W) is not supported");
-    if (s.neighborhood_flags & ~7u) return fail(CAVE_E_INVALID, "bad neighborhood flags");
+    // W: the cube search kernel ignores it (searchSimulationCube.slang:26-29), the ERD one wraps
+    if (s.neighborhood_flags & ~15u) return fail(CAVE_E_INVALID, "bad neighborhood flags");
+    if (s.flags & ~(CAVE_SIM_NO_EARLY_EXIT | CAVE_SIM_FINAL_HASH | CAVE_SIM_LEGACY_DEATH | CAVE_SIM_SHAPE_ERD))
+        return fail(CAVE_E_INVALID, "unknown sim flags");
     if (s.ticks == 0) return fail(CAVE_E_INVALID, "ticks must be >= 1");
     if (s.spawn_mode == CAVE_SPAWN_BUFFER || s.spawn_mode > CAVE_SPAWN_RENDER_RANDOM)
         return fail(CAVE_E_INVALID, "spawn_mode must be random, filled or render-random");
@@ -88,6 +90,20 @@ uint64_t fnv_ref_layout(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t 
     return h;
 }
 
+// FNV-1a-64 of the canonical byte stream, which is the ERD linear layout itself
+uint64_t fnv_linear(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z)
+{
+    const uint64_t n = (uint64_t)X * Y * Z;
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t i = 0; i < n; i += 2) {
+        uint32_t byte = (cells[i / 8u] >> (4u * (i % 8u))) & 15u;
+        if (i + 1 < n) byte |= ((cells[(i + 1) / 8u] >> (4u * ((i + 1) % 8u))) & 15u) << 4;
+        h ^= byte;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
 static uint32_t classify(const CaveGridInfo &info, uint32_t M)
 {   // searchSystem.cpp:695-711
     if (info.status == 2u) return 2u;
@@ -117,6 +133,14 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
     std::vector<StreamPlan> plans(n);
     for (size_t i = 0; i < n; i++) {
         GridGeom g{specs[i].dims[0], specs[i].dims[1], specs[i].dims[2], 4u * ((specs[i].dims[0] + 31u) / 32u), 1u};
+        if (specs[i].flags & CAVE_SIM_SHAPE_ERD) {  // linear layout, ERD kernel
+            g.planar = 0;
+            geo[i] = g;
+            use_stream[i] = 0;
+            max_parts = std::max(max_parts, erd_blocks(g.X, g.Y, g.Z));
+            max_words = std::max(max_words, cave_erd_cell_words(g.X, g.Y, g.Z));
+            continue;
+        }
         use_stream[i] = stream_supported(g);
         if (use_stream[i]) {
             plans[i] = stream_plan(g, 0, 0, 0);
@@ -151,17 +175,22 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
         const GridGeom &g = geo[i];
         const DevRule rule = make_rule(s.max_cell_state, s.neighborhood_flags, s.birth, s.survival);
         const DevInit init = make_init(s.dims, s.spawn, s.spawn_mode, s.density, s.seed);
+        const bool erd = (s.flags & CAVE_SIM_SHAPE_ERD) != 0;
+        const bool wrap = erd && (s.neighborhood_flags & CAVE_WRAP);
         uint32_t *bufs[2] = {b0.as<uint32_t>(), b1.as<uint32_t>()};
-        if ((rc = launch_init(bufs[0], g, init, rule.M, st))) return rc;
+        if ((rc = erd ? launch_init_erd(bufs[0], g.X, g.Y, g.Z, init, rule.M, st)
+                      : launch_init(bufs[0], g, init, rule.M, st)))
+            return rc;
         PlainState *state = dstate.as<PlainState>() + i;
         const uint32_t *done = &state->done;
         CaveTickStats *pt = want_pt ? dpt.as<CaveTickStats>() + i * stride : nullptr;
         for (uint32_t t = 0; t < s.ticks; t++) {
             uint32_t *in = bufs[t & 1u], *out = bufs[(t + 1u) & 1u];
-            rc = use_stream[i] ? launch_stream(in, out, g, rule, plans[i], dparts.as<StatPart>(), st, done)
-                               : launch_gather(in, out, g, rule, dparts.as<StatPart>(), st, done);
+            rc = erd            ? launch_erd(in, out, g.X, g.Y, g.Z, rule, wrap, dparts.as<StatPart>(), st, done)
+                 : use_stream[i] ? launch_stream(in, out, g, rule, plans[i], dparts.as<StatPart>(), st, done)
+                                 : launch_gather(in, out, g, rule, dparts.as<StatPart>(), st, done);
             if (rc) return rc;
-            const uint32_t nparts = use_stream[i] ? plans[i].blocks : gather_blocks(g);
+            const uint32_t nparts = erd ? erd_blocks(g.X, g.Y, g.Z) : use_stream[i] ? plans[i].blocks : gather_blocks(g);
             if ((rc = launch_plain_manager(dparts.as<StatPart>(), nparts, state, s.ticks, s.flags, pt,
                                            g.X * g.Y * g.Z, st)))
                 return rc;
@@ -173,6 +202,13 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
             std::vector<uint32_t> cells((size_t)upr * g.Y * g.Z);
             uint32_t *fin = bufs[hs[i].info.numberOfSimulationTicksSurvived & 1u];
             uint32_t *idle = bufs[(hs[i].info.numberOfSimulationTicksSurvived + 1u) & 1u];
+            if (erd) {
+                std::vector<uint32_t> lin(cave_erd_cell_words(g.X, g.Y, g.Z));
+                CAVE_HIP(hipMemcpyAsync(lin.data(), fin, lin.size() * 4, hipMemcpyDeviceToHost, st));
+                CAVE_HIP(hipStreamSynchronize(st));
+                hashes[i] = fnv_linear(lin.data(), g.X, g.Y, g.Z);
+                continue;
+            }
             if (g.planar) {
                 if ((rc = launch_planar_to_ref(fin, idle, g, st))) return rc;
                 CAVE_HIP(hipMemcpyAsync(cells.data(), idle, cells.size() * 4, hipMemcpyDeviceToHost, st));
@@ -208,6 +244,41 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
 
 std::string tuning_report();
 
+// Cube specs on the persistent engine, ERD specs on the per-config one;
+// results, per-tick rows and timing scattered back to spec order.
+int batch_mixed(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
+{
+    std::vector<size_t> idx[2];
+    for (size_t i = 0; i < n; i++) idx[(specs[i].flags & CAVE_SIM_SHAPE_ERD) ? 1 : 0].push_back(i);
+    const uint32_t stride = opt ? opt->per_tick_stride : 0u;
+    const bool want_pt = opt && opt->per_tick && stride;
+    double total_ms = 0;
+    for (int part = 0; part < 2; part++) {
+        const std::vector<size_t> &ix = idx[part];
+        std::vector<CaveSimSpec> sub(ix.size());
+        for (size_t k = 0; k < ix.size(); k++) sub[k] = specs[ix[k]];
+        std::vector<CaveSimResult> res(ix.size());
+        std::vector<CaveTickStats> pt(want_pt ? ix.size() * stride : 0);
+        std::vector<CaveSimTiming> tm(opt && opt->timing ? ix.size() : 0);
+        CaveBatchOptions o{};
+        if (opt) o = *opt;
+        o.per_tick = want_pt ? pt.data() : nullptr;
+        o.timing = tm.empty() ? nullptr : tm.data();
+        int rc = part == 0 ? batch_persistent(sub.data(), sub.size(), res.data(), &o, st)
+                           : batch_plain(sub.data(), sub.size(), res.data(), &o, st);
+        if (rc) return rc;
+        total_ms += o.device_ms;
+        if (part == 0 && opt) opt->block_threads = o.block_threads;
+        for (size_t k = 0; k < ix.size(); k++) {
+            results[ix[k]] = res[k];
+            if (want_pt) std::copy(pt.begin() + k * stride, pt.begin() + (k + 1) * stride, opt->per_tick + ix[k] * stride);
+            if (!tm.empty()) opt->timing[ix[k]] = tm[k];
+        }
+    }
+    if (opt) opt->device_ms = total_ms;
+    return CAVE_OK;
+}
+
 }  // namespace cave
 
 using namespace cave;
@@ -239,8 +310,17 @@ extern "C" int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimR
     if (engine > CAVE_ENGINE_PLAIN) return fail(CAVE_E_INVALID, "bad engine");
     hipStream_t st;
     CAVE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    int rc = engine == CAVE_ENGINE_PLAIN ? batch_plain(specs, n_specs, results, opt, st)
-                                         : batch_persistent(specs, n_specs, results, opt, st);
+    size_t n_erd = 0;
+    for (size_t i = 0; i < n_specs; i++) n_erd += (specs[i].flags & CAVE_SIM_SHAPE_ERD) ? 1 : 0;
+    int rc;
+    if (engine == CAVE_ENGINE_PLAIN || n_erd == n_specs) {
+        rc = batch_plain(specs, n_specs, results, opt, st);
+    } else if (n_erd == 0) {
+        rc = batch_persistent(specs, n_specs, results, opt, st);
+    } else {
+        // the persistent kernel runs cube cells; ERD specs take the per-config engine
+        rc = batch_mixed(specs, n_specs, results, opt, st);
+    }
     (void)hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     return rc;

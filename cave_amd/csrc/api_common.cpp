@@ -235,6 +235,11 @@ struct RenderGeom {
 
 }  // namespace
 
+extern "C" size_t cave_erd_cell_words(uint32_t X, uint32_t Y, uint32_t Z)
+{
+    return ((size_t)X * Y * Z + 7u) / 8u;
+}
+
 extern "C" size_t cave_render_bytes(uint32_t layout, uint32_t X, uint32_t Y, uint32_t Z)
 {
     const RenderGeom r(X, Y, Z);

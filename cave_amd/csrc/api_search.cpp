@@ -137,9 +137,8 @@ int cave_search_create(const CaveSearchSpec *spec, CaveSearch **out)
     if (!spec || !out) return fail(CAVE_E_INVALID, "null argument");
     *out = nullptr;
     const CaveChunkConfiguration &c = spec->chunk;
-    if (c.shapeId != 0) return fail(CAVE_E_UNSUPPORTED, "only the cube shape (id 0) is supported");
-    if (spec->neighborhood_flags & CAVE_WRAP)
-        return fail(CAVE_E_UNSUPPORTED, "wrap (W) is not supported by the cube search step");
+    if (c.shapeId > 1) return fail(CAVE_E_UNSUPPORTED, "shape ids are 0 (cube) and 1 (ERD)");
+    if (spec->neighborhood_flags & ~15u) return fail(CAVE_E_INVALID, "bad neighbourhood flags");
     if ((spec->neighborhood_flags & 7u) == 0) return fail(CAVE_E_INVALID, "neighbourhood must include F, E or C");
     if (c.gridDimensionX == 0 || c.gridDimensionY == 0 || c.gridDimensionZ == 0)
         return fail(CAVE_E_INVALID, "grid dimensions must be >= 1");
@@ -163,15 +162,25 @@ int cave_search_create(const CaveSearchSpec *spec, CaveSearch **out)
     for (int a = 0; a < 3; a++) h->spawn[a] = resolve_spawn(std::min(area[a], dims[a]), c.spawnSizeIndex);
     h->density = (double)((float)c.spawnDensityIndex + 1.0f) / 16.0;  // searchSystem.cpp:240
 
-    // slots: searchMode.cpp:49-102
+    // slots: searchMode.cpp:49-102 -- every non-empty subset of the selected
+    // F / E / C, each also with W when W is selected (W wraps on ERD; the cube
+    // search kernel ignores it, searchSimulationCube.slang:26-29)
     const uint32_t sel = spec->neighborhood_flags & 7u;
+    const bool erd = c.shapeId == 1u;
+    std::vector<uint32_t> configs;
+    for (uint32_t subset = 1; subset <= sel; subset++) {
+        if ((subset & sel) != subset) continue;
+        configs.push_back(subset);
+        if (spec->neighborhood_flags & CAVE_WRAP) configs.push_back(subset | CAVE_WRAP);
+    }
     for (int M = spec->min_max_cell_state; M <= spec->max_max_cell_state; M++)
-        for (uint32_t subset = 1; subset <= sel; subset++) {
-            if ((subset & sel) != subset) continue;
+        for (uint32_t subset : configs) {
             Slot s;
             s.M = (uint32_t)M;
             s.flags = subset;
-            const uint32_t n = neighbor_count(subset);
+            // neighbour counts per shape (src/shapes/*.h): ERD has 12 faces, 4 edges, no corners
+            const uint32_t n = erd ? ((subset & 1u) ? 12u : 0u) + ((subset & 2u) ? 4u : 0u) : neighbor_count(subset);
+            if (n == 0) continue;  // no neighbours: no rule space
             if (spec->max_rule_bits > 0) {
                 s.masks = enumerate_valid_bitmasks(n, spec->max_rule_bits);
                 s.max_perm = s.masks.size();
@@ -231,7 +240,8 @@ int cave_search_step(CaveSearch *h, uint64_t max_evals)
             sp.spawn_mode = h->spec.chunk.spawnRegionShapeId == 0 ? CAVE_SPAWN_RANDOM : CAVE_SPAWN_FILLED;
             sp.density = h->density;
             sp.seed = h->spec.chunk.runSeedIndex;
-            sp.flags = h->spec.sim_flags & ~(uint32_t)CAVE_SIM_FINAL_HASH;
+            sp.flags = h->spec.sim_flags & ~(uint32_t)(CAVE_SIM_FINAL_HASH | CAVE_SIM_SHAPE_ERD);
+            if (h->spec.chunk.shapeId == 1u) sp.flags |= CAVE_SIM_SHAPE_ERD;
             specs.push_back(sp);
             evals.push_back(e);
         }
@@ -247,7 +257,9 @@ int cave_search_step(CaveSearch *h, uint64_t max_evals)
             // bitmask form (searchSystem.cpp:668-685)
             r.permutation.birthAndMaxCellStateRules = ((uint64_t)((sp.max_cell_state - 1u) & 0xFu) << 60) | sp.birth;
             r.permutation.survivalAndNeighborhoodRules = ((uint64_t)sp.neighborhood_flags << 60) | sp.survival;
-            r.permutation.shapeAndGridConfiguration = h->spec.chunk.shapeId & 0x1Fu;
+            // the reference's chunks carry shape id 0 whatever the shader's shape
+            // (searchMode.cpp:197, searchSystem.cpp:198-199), so its records hold 0
+            r.permutation.shapeAndGridConfiguration = 0;
             r.ticksSurvived = res[i].info.numberOfSimulationTicksSurvived;
             r.totalBirths = res[i].info.totalNumberOfCellsThatWereBorn;
             r.totalDeaths = res[i].info.totalNumberOfCellsThatDied;

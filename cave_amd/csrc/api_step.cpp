@@ -31,6 +31,7 @@ struct CaveStep {
     CaveSimulationParameters params{};
     int kernel = CAVE_KERNEL_GATHER;
     bool wrap = false;
+    uint32_t shape = 0;  // 0 cube, 1 elongated rhombic dodecahedron (linear layout)
     StreamPlan plan{};
     uint32_t nparts = 0;
     hipStream_t stream = nullptr;
@@ -53,12 +54,12 @@ namespace {
 
 constexpr uint32_t kChunkTicks = 1024;
 
-int parse_rule(const CaveSimulationParameters &p, DevRule *out, bool *wrap)
+int parse_rule(const CaveSimulationParameters &p, DevRule *out, bool *wrap, uint32_t *shape)
 {
     const uint32_t flags = uint32_t(p.survivalAndNeighborhoodRules >> 60);
     *wrap = (flags & CAVE_WRAP) != 0;
-    if ((p.shapeAndGridConfiguration & 0x1Fu) != 0)
-        return fail(CAVE_E_UNSUPPORTED, "only the cube shape (id 0) is supported");
+    *shape = uint32_t(p.shapeAndGridConfiguration & 0x1Fu);  // searchSystem.cpp:198-199
+    if (*shape > 1u) return fail(CAVE_E_UNSUPPORTED, "shape ids are 0 (cube) and 1 (ERD)");
     const uint32_t M = uint32_t(p.birthAndMaxCellStateRules >> 60) + 1u;
     *out = make_rule(M, flags, p.birthAndMaxCellStateRules & kPermMask, p.survivalAndNeighborhoodRules & kPermMask);
     return CAVE_OK;
@@ -75,7 +76,9 @@ int launch_ticks(CaveStep *h, uint32_t n, uint32_t *cur, uint32_t *nxt)
 {
     for (uint32_t t = 0; t < n; t++) {
         StatPart *parts = h->partials + (size_t)t * h->nparts;
-        int rc = h->kernel != CAVE_KERNEL_GATHER
+        int rc = h->kernel == CAVE_KERNEL_ERD
+                     ? launch_erd(cur, nxt, h->g.X, h->g.Y, h->g.Z, h->rule, h->wrap, parts, h->stream)
+                 : h->kernel != CAVE_KERNEL_GATHER
                      ? launch_stream(cur, nxt, h->g, h->rule, h->plan, parts, h->stream)
                      : launch_gather(cur, nxt, h->g, h->rule, parts, h->stream, nullptr, h->wrap);
         if (rc) return rc;
@@ -125,7 +128,11 @@ void release(CaveStep *h)
     delete h;
 }
 
-size_t dev_words(const GridGeom &g) { return (size_t)g.pitch * g.Y * g.Z; }
+size_t dev_words(const CaveStep *h)
+{
+    if (h->shape == 1u) return cave_erd_cell_words(h->g.X, h->g.Y, h->g.Z);
+    return (size_t)h->g.pitch * h->g.Y * h->g.Z;
+}
 
 }  // namespace
 
@@ -144,12 +151,16 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     if (spec->spawn_mode > CAVE_SPAWN_RENDER_RANDOM) return fail(CAVE_E_INVALID, "bad spawn_mode");
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER && !spec->initial_cells)
         return fail(CAVE_E_INVALID, "CAVE_SPAWN_BUFFER needs initial_cells");
-    if (spec->kernel > CAVE_KERNEL_STREAM_NIBBLE) return fail(CAVE_E_INVALID, "bad kernel selector");
+    if (spec->kernel > CAVE_KERNEL_ERD) return fail(CAVE_E_INVALID, "bad kernel selector");
 
     CaveStep *h = new (std::nothrow) CaveStep();
     if (!h) return fail(CAVE_E_NOMEM, "host allocation failed");
-    int rc = parse_rule(spec->params, &h->rule, &h->wrap);
+    int rc = parse_rule(spec->params, &h->rule, &h->wrap, &h->shape);
     if (rc) { release(h); return rc; }
+    if ((h->shape == 1u) != (spec->kernel == CAVE_KERNEL_ERD) && spec->kernel != CAVE_KERNEL_AUTO) {
+        release(h);
+        return fail(CAVE_E_INVALID, "CAVE_KERNEL_ERD runs exactly the ERD shape (id 1)");
+    }
     h->params = spec->params;
     h->g.X = spec->dims[0];
     h->g.Y = spec->dims[1];
@@ -157,16 +168,20 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
 
     GridGeom sg = h->g;
     sg.pitch = 4u * ((h->g.X + 31u) / 32u);
-    const bool stream_ok = stream_supported(sg) && !h->wrap;
+    const bool stream_ok = stream_supported(sg) && !h->wrap && h->shape == 0u;
     const bool want_stream = spec->kernel == CAVE_KERNEL_STREAM || spec->kernel == CAVE_KERNEL_STREAM_NIBBLE;
     if (want_stream && !stream_ok) {
         release(h);
         return fail(CAVE_E_UNSUPPORTED, h->wrap ? "wrapping grids run on the gather kernel"
                                                 : "the streaming kernel needs X <= 512");
     }
-    if (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) h->kernel = CAVE_KERNEL_GATHER;
+    if (h->shape == 1u) h->kernel = CAVE_KERNEL_ERD;
+    else if (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) h->kernel = CAVE_KERNEL_GATHER;
     else h->kernel = spec->kernel == CAVE_KERNEL_STREAM_NIBBLE ? CAVE_KERNEL_STREAM_NIBBLE : CAVE_KERNEL_STREAM;
-    if (h->kernel != CAVE_KERNEL_GATHER) {
+    if (h->kernel == CAVE_KERNEL_ERD) {
+        h->g.pitch = 0;
+        h->nparts = erd_blocks(h->g.X, h->g.Y, h->g.Z);
+    } else if (h->kernel != CAVE_KERNEL_GATHER) {
         sg.planar = h->kernel == CAVE_KERNEL_STREAM ? 1u : 0u;
         h->g = sg;
         // CAVE_STREAM_PLAN="rows_per_lane,z_planes,waves" overrides the default tiling
@@ -179,7 +194,7 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
         h->nparts = gather_blocks(h->g);
     }
 
-    const size_t bytes = dev_words(h->g) * sizeof(uint32_t);
+    const size_t bytes = dev_words(h) * sizeof(uint32_t);
 #define CK(expr)                                                                 \
     do {                                                                         \
         hipError_t _e = (expr);                                                  \
@@ -205,12 +220,13 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
         CK(hipStreamSynchronize(h->stream));
-        rc = cave_step_upload(h, spec->initial_cells, cave_cell_words(h->g.X, h->g.Y, h->g.Z));
+        rc = cave_step_upload(h, spec->initial_cells, dev_words(h));
         if (rc) { release(h); return rc; }
     } else {
         const uint32_t dims[3] = {h->g.X, h->g.Y, h->g.Z};
         const DevInit init = make_init(dims, spec->spawn, spec->spawn_mode, spec->density, spec->seed);
-        rc = launch_init(h->cur, h->g, init, h->rule.M, h->stream);
+        rc = h->shape == 1u ? launch_init_erd(h->cur, h->g.X, h->g.Y, h->g.Z, init, h->rule.M, h->stream)
+                            : launch_init(h->cur, h->g, init, h->rule.M, h->stream);
         if (rc) { release(h); return rc; }
     }
     CK(hipStreamSynchronize(h->stream));
@@ -258,6 +274,12 @@ int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *
 int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words)
 {
     if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
+    if (h->shape == 1u) {  // the ERD linear layout is stored as is
+        if (n_words < dev_words(h)) return fail(CAVE_E_INVALID, "buffer too small");
+        CAVE_HIP(hipMemcpyAsync(cells, h->cur, dev_words(h) * 4, hipMemcpyDeviceToHost, h->stream));
+        CAVE_HIP(hipStreamSynchronize(h->stream));
+        return CAVE_OK;
+    }
     const uint32_t upr = ref_words_per_row(h->g.X);
     if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
     if (h->g.planar) {
@@ -276,6 +298,12 @@ int cave_step_download(CaveStep *h, uint32_t *cells, size_t n_words)
 int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words)
 {
     if (!h || !cells) return fail(CAVE_E_INVALID, "null argument");
+    if (h->shape == 1u) {
+        if (n_words < dev_words(h)) return fail(CAVE_E_INVALID, "buffer too small");
+        CAVE_HIP(hipMemcpyAsync(h->cur, cells, dev_words(h) * 4, hipMemcpyHostToDevice, h->stream));
+        CAVE_HIP(hipStreamSynchronize(h->stream));
+        return CAVE_OK;
+    }
     const uint32_t upr = ref_words_per_row(h->g.X);
     if (n_words < (size_t)upr * h->g.Y * h->g.Z) return fail(CAVE_E_INVALID, "buffer too small");
     if (h->g.planar) {
@@ -283,7 +311,7 @@ int cave_step_upload(CaveStep *h, const uint32_t *cells, size_t n_words)
         int rc = launch_ref_to_planar(h->nxt, h->cur, h->g, h->stream);
         if (rc) return rc;
     } else {
-        CAVE_HIP(hipMemsetAsync(h->cur, 0, dev_words(h->g) * 4, h->stream));
+        CAVE_HIP(hipMemsetAsync(h->cur, 0, dev_words(h) * 4, h->stream));
         CAVE_HIP(hipMemcpy2DAsync(h->cur, (size_t)h->g.pitch * 4, cells, (size_t)upr * 4, (size_t)upr * 4,
                                   (size_t)h->g.Y * h->g.Z, hipMemcpyHostToDevice, h->stream));
         int rc = launch_sanitize_nibbles(h->cur, h->g, h->stream);
@@ -298,10 +326,11 @@ int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params)
     if (!h || !params) return fail(CAVE_E_INVALID, "null argument");
     DevRule r{};
     bool wrap = false;
-    int rc = parse_rule(*params, &r, &wrap);
+    uint32_t shape = 0;
+    int rc = parse_rule(*params, &r, &wrap, &shape);
     if (rc) return rc;
-    if (wrap != h->wrap)
-        return fail(CAVE_E_STATE, "the W flag cannot change on a live handle (it selects the kernel)");
+    if (wrap != h->wrap || shape != h->shape)
+        return fail(CAVE_E_STATE, "the W flag and the shape cannot change on a live handle (they select the kernel)");
     drop_graphs(h);  // the rule is baked into the captured launches
     h->rule = r;
     h->params = *params;

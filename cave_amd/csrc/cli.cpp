@@ -4,6 +4,7 @@
 // codes follow src/main.cpp:21-221, 343-377: 0 ok, 1 validation error,
 // 2 argument parse error, 4 exception.
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -100,7 +101,8 @@ const char *kHelp =
     "          --min-cs M0 --max-cs-range M1, --ticks T, --chunks-per-config P, --spawn S,\n"
     "          --spawn-mode random|filled, --spawn-density d | --density-sweep a:b:s,\n"
     "          --seed-runs R, --output DIR, --legacy-death\n"
-    "          (accepted and ignored: --headless --dual-gpu --search-workgroup-size --gpu-vram-budget-gb --shape cube)\n"
+    "          --shape cube|ed (W in --neighborhood adds wrapping slots)\n"
+    "          (accepted and ignored: --headless --dual-gpu --search-workgroup-size --gpu-vram-budget-gb)\n"
     "  step:   --grid N, --birth \"5,6\", --survival \"4-6\", --max-cs M, --neighborhood FE[W], --frames T,\n"
     "          --spawn S, --spawn-mode random|filled, --spawn-density d, --seed s, --dump FILE\n"
     "          (--mode render --headless: the same loop with the render path's unsalted random spawn;\n"
@@ -164,19 +166,20 @@ int check(int rc)
 
 int run_search(const Args &a)
 {
-    const std::string shape = a.str("shape", "cube");
-    if (shape != "cube") {
-        log_error("Unknown or unsupported --shape: " + shape + " (this build runs the cube shape)");
+    // src/main.cpp:248-258: cube, or ed / elongateddodecahedron (case-insensitive)
+    std::string shape = a.str("shape", "cube");
+    std::transform(shape.begin(), shape.end(), shape.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
+    uint32_t shape_id = 0;
+    if (shape == "cube") shape_id = 0;
+    else if (shape == "ed" || shape == "elongateddodecahedron") shape_id = 1;
+    else {
+        log_error("Unknown --shape: " + a.str("shape", "") + ". Use 'cube' or 'ed'.");
         return 1;
     }
     const std::string nb = a.str("neighborhood", "F");
     const uint32_t sel = parse_flags(nb, true);
     if (!(sel & 7u)) {
         log_error("--neighborhood must include at least one of F / E / C (got \"" + nb + "\")");
-        return 1;
-    }
-    if (sel & CAVE_WRAP) {
-        log_error("--neighborhood W (wrap) is not supported by the cube search kernel");
         return 1;
     }
     const int min_cs = a.integer("min-cs", 3), max_cs = a.integer("max-cs-range", 10);
@@ -232,11 +235,13 @@ int run_search(const Args &a)
     if (sel & CAVE_FACES) tag += "F";
     if (sel & CAVE_EDGES) tag += "E";
     if (sel & CAVE_CORNERS) tag += "C";
+    if (sel & CAVE_WRAP) tag += "W";
     std::string base = "cs" + std::to_string(min_cs) + "-" + std::to_string(max_cs) + "_" + tag + "_t" + std::to_string(ticks);
     if (K > 0) base += "_b" + std::to_string(K);
     std::string out = a.str("output", "data/search/");
     if (!out.empty() && out.back() != '/') out += "/";
-    const std::string folder = out + "cube/";
+    // searchMode.cpp:120: <output><shape name>/
+    const std::string folder = out + (shape_id == 1 ? "ElongatedRhombicDodecahedron/" : "cube/");
     std::filesystem::create_directories(folder);
     const uint32_t sim_flags = a.flag("legacy-death") ? CAVE_SIM_LEGACY_DEATH : 0u;
 
@@ -253,6 +258,7 @@ int run_search(const Args &a)
                 for (int p = 0; p < P; p++) {
                     CaveSearchSpec spec{};
                     CaveChunkConfiguration &c = spec.chunk;
+                    c.shapeId = shape_id;
                     c.gridDimensionX = c.gridDimensionY = c.gridDimensionZ = g;
                     c.spawnAreaDimensionX = c.spawnAreaDimensionY = c.spawnAreaDimensionZ = std::min((uint32_t)spawn, g);
                     c.spawnRegionShapeId = random ? 0u : 1u;

@@ -32,6 +32,13 @@ int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st);
 uint32_t gather_blocks(const GridGeom &g);
 // done (nullable): device flag; when set the launch does nothing (the plain
 // per-config driver ends a simulation on the device without a host round trip)
+// ERD shape, linear layout (step_erd.hip)
+uint32_t erd_blocks(uint32_t X, uint32_t Y, uint32_t Z);
+int launch_erd(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z, const DevRule &rule, bool wrap,
+               StatPart *partials, hipStream_t st, const uint32_t *done = nullptr);
+int launch_init_erd(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, const DevInit &init, uint32_t M,
+                    hipStream_t st);
+
 // wrap: every axis wraps (the render path's W flag)
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
                   hipStream_t st, const uint32_t *done = nullptr, bool wrap = false);

@@ -117,6 +117,11 @@ int cave_device_info(char *name, size_t cap, int *cu_count, uint64_t *hbm_bytes)
 uint64_t cave_rule_string_to_bitmask(const char *rule);
 size_t cave_bitmask_to_rule_string(uint64_t bitmask, char *out, size_t cap);
 size_t cave_enumerate_valid_bitmasks(uint32_t neighbor_count, uint32_t max_bits, uint64_t *out, size_t cap);
+/* words of the ERD shape's linear layout (cell x + X*(y + Y*z), 8 per word,
+ * searchSystem.cpp:296-300): ceil(X*Y*Z/8).  A step handle whose parameters
+ * carry shape id 1 (shapeAndGridConfiguration bits 0-4) uploads / downloads
+ * this layout. */
+size_t cave_erd_cell_words(uint32_t X, uint32_t Y, uint32_t Z);
 /* words of the reference packed layout: ceil(X/8) * Y * Z */
 size_t cave_cell_words(uint32_t X, uint32_t Y, uint32_t Z);
 /* splitmix64 generator used for every synthetic workload (seeded, portable) */
@@ -143,7 +148,8 @@ enum {
     CAVE_KERNEL_AUTO = 0,
     CAVE_KERNEL_GATHER = 1,        /* one lane per packed word, per-cell gather            */
     CAVE_KERNEL_STREAM = 2,        /* bit-sliced z-streaming kernel, bit-planar cell groups */
-    CAVE_KERNEL_STREAM_NIBBLE = 3  /* the same kernel on the reference nibble layout        */
+    CAVE_KERNEL_STREAM_NIBBLE = 3, /* the same kernel on the reference nibble layout        */
+    CAVE_KERNEL_ERD = 4            /* the ERD shape (shape id 1), linear layout              */
 };
 
 typedef struct CaveStepSpec {
@@ -210,6 +216,11 @@ typedef struct CaveSimSpec {
  * shaders/source/search/manager.comp:115) instead of "no survivor and no
  * birth" (shaders/slang/search/searchManager.slang:128) */
 #define CAVE_SIM_LEGACY_DEATH 4u
+/* the elongated rhombic dodecahedron cell shape (shaders/slang/search/
+ * searchSimulationERD.slang): 12 face neighbours by y parity + 4 edge
+ * neighbours (C ignored), W wraps every axis.  Cube specs ignore W, as the
+ * cube search kernel does (searchSimulationCube.slang:26-29). */
+#define CAVE_SIM_SHAPE_ERD 8u
 
 typedef struct CaveSimResult {
     CaveGridInfo info;       /* GridInfo at completion (manager semantics) */
@@ -259,8 +270,10 @@ int cave_batch_tuning_report(char *buf, size_t cap);
 typedef struct CaveSearch CaveSearch;
 
 typedef struct CaveSearchSpec {
-    CaveChunkConfiguration chunk;
-    uint32_t neighborhood_flags; /* selected F|E|C; every non-empty subset is a slot */
+    CaveChunkConfiguration chunk; /* chunk.shapeId selects the cell shape: 0 cube, 1 ERD (records
+                                     carry shapeAndGridConfiguration 0 either way, as the reference's) */
+    uint32_t neighborhood_flags; /* selected F|E|C (+W); every non-empty subset is a slot, and with W
+                                    also the subset with W (searchMode.cpp:56-63) */
     uint32_t max_rule_bits;      /* K (0 = raw bitmask space) */
     int32_t min_max_cell_state;  /* --min-cs */
     int32_t max_max_cell_state;  /* --max-cs-range */

@@ -265,6 +265,131 @@ void orc_step_render(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, 
 }
 
 /* ------------------------------------------------------------------ */
+/* ERD step: shaders/slang/search/searchSimulationERD.slang             */
+/* ------------------------------------------------------------------ */
+/* linear nibble layout (searchSystem.cpp:296-300): cell x + X*(y + Y*z), 8 per word */
+size_t orc_erd_words(uint32_t X, uint32_t Y, uint32_t Z)
+{
+    return ((size_t)X * Y * Z + 7u) / 8u;
+}
+
+static uint32_t erd_get(const uint32_t *c, size_t i) { return (c[i / 8u] >> ((i % 8u) * 4u)) & 0xFu; }
+
+/* shaders/slang/common/elongatedRhombicDodecahedron.slang:6-27 */
+static const int kErdFace[2][12][3] = {
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},
+     {-1, -1, 0}, {-1, -1, -1}, {0, -1, -1}, {-1, 1, 0}, {-1, 1, -1}, {0, 1, -1}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},
+     {1, -1, 0}, {1, -1, 1}, {0, -1, 1}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}},
+};
+static const int kErdEdge[4][3] = {{1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+
+uint32_t orc_erd_neighbor_count(uint32_t flags) { return ((flags & 1u) ? 12u : 0u) + ((flags & 2u) ? 4u : 0u); }
+
+/* searchSimulationERD.slang:59-80: wrap (W) or bounds check, linear read */
+static int erd_alive(const uint32_t *in, int x, int y, int z, uint32_t M, uint32_t X, uint32_t Y, uint32_t Z,
+                     int wrap)
+{
+    if (wrap) {
+        x = (x + (int)X) % (int)X;
+        y = (y + (int)Y) % (int)Y;
+        z = (z + (int)Z) % (int)Z;
+    } else if (x < 0 || x >= (int)X || y < 0 || y >= (int)Y || z < 0 || z >= (int)Z) {
+        return 0;
+    }
+    return erd_get(in, (size_t)x + (size_t)X * ((size_t)y + (size_t)Y * (size_t)z)) == M;
+}
+
+void orc_step_erd(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z, uint64_t word0,
+                  uint64_t word1, const uint64_t *valid_bitmasks, uint32_t max_rule_bits, orc_tick_stats *st)
+{   /* searchSimulationERD.slang:99-217 */
+    const uint32_t M = (uint32_t)(word0 >> 60) + 1u;
+    const uint32_t flags = (uint32_t)(word1 >> 60);
+    const int wrap = (flags & 8u) != 0;
+    uint64_t B, S;
+    if (max_rule_bits > 0) {
+        B = valid_bitmasks[(uint32_t)(word0 & PERM_MASK) - 1u];
+        S = valid_bitmasks[(uint32_t)(word1 & PERM_MASK) - 1u];
+    } else {
+        B = word0 & PERM_MASK;
+        S = word1 & PERM_MASK;
+    }
+    memset(st, 0, sizeof(*st));
+    for (int a = 0; a < 3; a++) { st->bbox_min[a] = 0x7fffffff; st->bbox_max[a] = -1; }
+    const size_t nw = orc_erd_words(X, Y, Z);
+    memset(out, 0, nw * sizeof(uint32_t));
+    for (uint32_t z = 0; z < Z; z++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t x = 0; x < X; x++) {
+                const size_t i = (size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z);
+                const uint32_t cur = erd_get(in, i);
+                const int cfg = (int)(y & 1u);
+                int n = 0;
+                if (flags & 1u)
+                    for (int k = 0; k < 12; k++)
+                        n += erd_alive(in, (int)x + kErdFace[cfg][k][0], (int)y + kErdFace[cfg][k][1],
+                                       (int)z + kErdFace[cfg][k][2], M, X, Y, Z, wrap);
+                if (flags & 2u)
+                    for (int k = 0; k < 4; k++)
+                        n += erd_alive(in, (int)x + kErdEdge[k][0], (int)y + kErdEdge[k][1], (int)z + kErdEdge[k][2],
+                                       M, X, Y, Z, wrap);
+                uint32_t ns;
+                if (cur == M) {
+                    if (n > 0 && ((S >> (uint64_t)(n - 1)) & 1u)) { ns = M; st->snap.numberOfSurvivingCells++; }
+                    else { ns = cur - 1u; st->snap.numberOfDyingCells++; }
+                } else if (cur == 0u) {
+                    if (n > 0 && ((B >> (uint64_t)(n - 1)) & 1u)) { ns = M; st->snap.numberOfCellsBorn++; }
+                    else { ns = 0u; st->snap.numberOfDeadCells++; }
+                } else {
+                    ns = cur - 1u;
+                    st->snap.numberOfDyingCells++;
+                }
+                out[i / 8u] |= (ns & 0xFu) << ((i % 8u) * 4u);
+                if (ns == M) {
+                    const int c[3] = {(int)x, (int)y, (int)z};
+                    for (int a = 0; a < 3; a++) {
+                        if (c[a] < st->bbox_min[a]) st->bbox_min[a] = c[a];
+                        if (c[a] > st->bbox_max[a]) st->bbox_max[a] = c[a];
+                    }
+                }
+            }
+    st->live = st->snap.numberOfCellsBorn + st->snap.numberOfSurvivingCells;
+    st->changed = st->snap.numberOfCellsBorn + st->snap.numberOfDyingCells;
+}
+
+void orc_init_cells_erd(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state, uint32_t sx,
+                        uint32_t sy, uint32_t sz, uint32_t spawn_mode, double density, uint32_t seed)
+{   /* searchSystem.cpp:264-303, linear branch */
+    memset(cells, 0, orc_erd_words(X, Y, Z) * sizeof(uint32_t));
+    const uint32_t threshold = orc_density_threshold(density);
+    for (uint32_t z = 0; z < Z; z++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t x = 0; x < X; x++) {
+                int in_box = x >= (X - sx) / 2 && x < (X + sx) / 2 && y >= (Y - sy) / 2 && y < (Y + sy) / 2 &&
+                             z >= (Z - sz) / 2 && z < (Z + sz) / 2;
+                if (!in_box) continue;
+                int alive = spawn_mode == 0 ? (density >= 1.0 || orc_cell_hash(x, y, z, seed) < threshold) : 1;
+                if (alive) {
+                    const size_t i = (size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z);
+                    cells[i / 8u] |= (max_cell_state & 0xFu) << ((i % 8u) * 4u);
+                }
+            }
+}
+
+uint64_t orc_erd_canonical_hash(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z)
+{   /* the canonical hash's byte stream is the linear layout itself */
+    const uint64_t n = (uint64_t)X * Y * Z;
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t i = 0; i < n; i += 2) {
+        uint32_t byte = erd_get(cells, i);
+        if (i + 1 < n) byte |= erd_get(cells, i + 1) << 4;
+        h ^= byte;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+/* ------------------------------------------------------------------ */
 /* manager: shaders/slang/search/searchManager.slang                   */
 /* ------------------------------------------------------------------ */
 static void next_permutation(orc_grid_info *info, orc_permutation *p, uint64_t max_permutation,
@@ -479,7 +604,15 @@ static uint32_t run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_s
                          uint32_t flags, uint64_t birth, uint64_t survival,
                          uint32_t max_ticks, const uint32_t *initial_cells,
                          orc_grid_info *info_out, orc_tick_stats *per_tick,
-                         uint32_t *final_cells, int legacy_death);
+                         uint32_t *final_cells, int legacy_death, int erd);
+
+uint32_t orc_run_rule_erd(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state, uint32_t flags,
+                          uint64_t birth, uint64_t survival, uint32_t max_ticks, const uint32_t *initial_cells,
+                          orc_grid_info *info_out, orc_tick_stats *per_tick, uint32_t *final_cells)
+{
+    return run_rule(X, Y, Z, max_cell_state, flags, birth, survival, max_ticks, initial_cells, info_out, per_tick,
+                    final_cells, 0, 1);
+}
 
 uint32_t orc_run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state,
                       uint32_t flags, uint64_t birth, uint64_t survival,
@@ -488,16 +621,16 @@ uint32_t orc_run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_stat
                       uint32_t *final_cells)
 {
     return run_rule(X, Y, Z, max_cell_state, flags, birth, survival, max_ticks, initial_cells,
-                    info_out, per_tick, final_cells, 0);
+                    info_out, per_tick, final_cells, 0, 0);
 }
 
 static uint32_t run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state,
                          uint32_t flags, uint64_t birth, uint64_t survival,
                          uint32_t max_ticks, const uint32_t *initial_cells,
                          orc_grid_info *info_out, orc_tick_stats *per_tick,
-                         uint32_t *final_cells, int legacy_death)
+                         uint32_t *final_cells, int legacy_death, int erd)
 {
-    size_t nw = orc_cell_words(X, Y, Z);
+    size_t nw = erd ? orc_erd_words(X, Y, Z) : orc_cell_words(X, Y, Z);
     uint32_t *a = (uint32_t *)malloc(nw * sizeof(uint32_t));
     uint32_t *b = (uint32_t *)malloc(nw * sizeof(uint32_t));
     memcpy(a, initial_cells, nw * sizeof(uint32_t));
@@ -511,8 +644,12 @@ static uint32_t run_rule(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_s
     uint32_t t = 0;
     while (1) {
         orc_tick_stats st;
-        step_impl(a, b, X, Y, Z, perm.birthAndMaxCellStateRules, perm.survivalAndNeighborhoodRules,
-                  NULL, 0, &st, 0);
+        if (erd)
+            orc_step_erd(a, b, X, Y, Z, perm.birthAndMaxCellStateRules, perm.survivalAndNeighborhoodRules, NULL, 0,
+                         &st);
+        else
+            step_impl(a, b, X, Y, Z, perm.birthAndMaxCellStateRules, perm.survivalAndNeighborhoodRules,
+                      NULL, 0, &st, 0);
         if (per_tick && t < max_ticks) per_tick[t] = st;
         t++;
         /* the copy-based ping-pong of searchSystem.cpp:599-616 */
@@ -547,6 +684,7 @@ typedef struct {
     uint32_t max_ticks;
     uint32_t **initial; /* indexed by M */
     int legacy_death;
+    int erd;
     uint64_t counts[3];
 } search_ctx;
 
@@ -565,7 +703,7 @@ static void *search_worker(void *arg)
             const orc_job *j = &c->jobs[i];
             orc_grid_info info;
             run_rule(c->X, c->X, c->X, j->M, j->flags, j->birth, j->survival, c->max_ticks,
-                     c->initial[j->M], &info, NULL, NULL, c->legacy_death);
+                     c->initial[j->M], &info, NULL, NULL, c->legacy_death, c->erd);
             local[orc_classify(&info, j->M)]++;
         }
     }
@@ -589,18 +727,29 @@ uint64_t orc_run_search_ex(uint32_t grid, uint32_t sel_flags, uint32_t max_rule_
                            uint32_t spawn, uint32_t spawn_mode, double density,
                            uint32_t seed, int n_threads, uint64_t counts[3], int legacy_death)
 {
+    return orc_run_search_shape(grid, 0, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn, spawn_mode,
+                                density, seed, n_threads, counts, legacy_death);
+}
+
+uint64_t orc_run_search_shape(uint32_t grid, uint32_t shape, uint32_t sel_flags, uint32_t max_rule_bits,
+                              int min_m, int max_m, uint32_t max_ticks, uint32_t spawn, uint32_t spawn_mode,
+                              double density, uint32_t seed, int n_threads, uint64_t counts[3], int legacy_death)
+{   /* shape 0: cube (3D-tiled, padded rows); shape 1: ERD (linear layout), searchMode.cpp:71-76 */
+    const int erd = shape == 1u;
     /* slots: searchMode.cpp:49-102 (subsets of the selected F/E/C flags) */
     size_t cap = 0, n = 0;
     orc_job *jobs = NULL;
     uint32_t *initial[17] = {0};
-    size_t nw = orc_cell_words(grid, grid, grid);
+    size_t nw = erd ? orc_erd_words(grid, grid, grid) : orc_cell_words(grid, grid, grid);
     for (int M = min_m; M <= max_m; M++) {
         initial[M] = (uint32_t *)malloc(nw * sizeof(uint32_t));
         uint32_t s = spawn < grid ? spawn : grid; /* searchMode.cpp:177-179 */
-        orc_init_cells(initial[M], grid, grid, grid, (uint32_t)M, s, s, s, spawn_mode, density, seed);
+        if (erd) orc_init_cells_erd(initial[M], grid, grid, grid, (uint32_t)M, s, s, s, spawn_mode, density, seed);
+        else orc_init_cells(initial[M], grid, grid, grid, (uint32_t)M, s, s, s, spawn_mode, density, seed);
         for (uint32_t subset = 1; subset <= sel_flags; subset++) {
             if ((subset & sel_flags) != subset) continue;
-            uint32_t N = orc_neighbor_count(subset);
+            uint32_t N = erd ? orc_erd_neighbor_count(subset) : orc_neighbor_count(subset);
+            if (N == 0) continue; /* no neighbours (corners only on ERD): no rule space */
             uint64_t *vb = NULL;
             uint64_t maxp;
             if (max_rule_bits > 0) {
@@ -632,6 +781,7 @@ uint64_t orc_run_search_ex(uint32_t grid, uint32_t sel_flags, uint32_t max_rule_
     c.max_ticks = max_ticks;
     c.initial = initial;
     c.legacy_death = legacy_death;
+    c.erd = erd;
     pthread_mutex_init(&c.mu, NULL);
     if (n_threads < 1) n_threads = 1;
     pthread_t th[256];

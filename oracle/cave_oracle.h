@@ -108,6 +108,21 @@ uint32_t orc_render_density_threshold(double density, int *all_alive);
 void orc_init_cells_render(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state,
                            uint32_t sx, uint32_t sy, uint32_t sz, uint32_t spawn_mode, double density);
 
+/* ERD shape (searchSimulationERD.slang): linear layout, 12 face / 4 edge deltas */
+size_t orc_erd_words(uint32_t X, uint32_t Y, uint32_t Z);
+uint32_t orc_erd_neighbor_count(uint32_t flags);
+void orc_step_erd(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z, uint64_t word0,
+                  uint64_t word1, const uint64_t *valid_bitmasks, uint32_t max_rule_bits, orc_tick_stats *st);
+void orc_init_cells_erd(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state, uint32_t sx,
+                        uint32_t sy, uint32_t sz, uint32_t spawn_mode, double density, uint32_t seed);
+uint64_t orc_erd_canonical_hash(const uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z);
+uint32_t orc_run_rule_erd(uint32_t X, uint32_t Y, uint32_t Z, uint32_t max_cell_state, uint32_t flags,
+                          uint64_t birth, uint64_t survival, uint32_t max_ticks, const uint32_t *initial_cells,
+                          orc_grid_info *info_out, orc_tick_stats *per_tick, uint32_t *final_cells);
+uint64_t orc_run_search_shape(uint32_t grid, uint32_t shape, uint32_t sel_flags, uint32_t max_rule_bits,
+                              int min_m, int max_m, uint32_t max_ticks, uint32_t spawn, uint32_t spawn_mode,
+                              double density, uint32_t seed, int n_threads, uint64_t counts[3], int legacy_death);
+
 void orc_step(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z,
               uint64_t word0, uint64_t word1, const uint64_t *valid_bitmasks,
               uint32_t max_rule_bits, orc_snapshot *snap);

@@ -103,6 +103,23 @@ def lib():
         L.orc_run_search.restype = u64
         L.orc_run_search.argtypes = [u32, u32, u32, C.c_int, C.c_int, u32, u32, u32, C.c_double, u32,
                                      C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_erd_words.restype = C.c_size_t
+        L.orc_erd_words.argtypes = [u32, u32, u32]
+        L.orc_erd_neighbor_count.restype = u32
+        L.orc_erd_neighbor_count.argtypes = [u32]
+        L.orc_step_erd.restype = None
+        L.orc_step_erd.argtypes = [p32, p32, u32, u32, u32, u64, u64, C.POINTER(C.c_uint64), u32,
+                                   C.POINTER(TickStats)]
+        L.orc_init_cells_erd.restype = None
+        L.orc_init_cells_erd.argtypes = [p32, u32, u32, u32, u32, u32, u32, u32, u32, C.c_double, u32]
+        L.orc_erd_canonical_hash.restype = u64
+        L.orc_erd_canonical_hash.argtypes = [p32, u32, u32, u32]
+        L.orc_run_rule_erd.restype = u32
+        L.orc_run_rule_erd.argtypes = [u32, u32, u32, u32, u32, u64, u64, u32, p32, C.POINTER(GridInfo),
+                                       C.POINTER(TickStats), p32]
+        L.orc_run_search_shape.restype = u64
+        L.orc_run_search_shape.argtypes = [u32, u32, u32, u32, C.c_int, C.c_int, u32, u32, u32, C.c_double, u32,
+                                           C.c_int, C.POINTER(C.c_uint64), C.c_int]
         L.orc_step_render.restype = None
         L.orc_step_render.argtypes = [p32, p32, u32, u32, u32, u64, u64, C.POINTER(TickStats)]
         L.orc_render_density_threshold.restype = u32
@@ -127,6 +144,50 @@ def init_cells(X, Y, Z, M, spawn, spawn_mode, density, seed) -> np.ndarray:
     sx, sy, sz = spawn
     lib().orc_init_cells(_p32(a), X, Y, Z, M, sx, sy, sz, spawn_mode, float(density), seed)
     return a
+
+
+# ---- ERD shape (linear layout) ----
+def erd_words(X, Y, Z) -> int:
+    return int(lib().orc_erd_words(X, Y, Z))
+
+
+def step_erd(cells, X, Y, Z, word0, word1, valid_bitmasks=None, max_rule_bits=0):
+    out = np.zeros_like(cells)
+    st = TickStats()
+    vb = None
+    if valid_bitmasks is not None:
+        vb_arr = np.ascontiguousarray(valid_bitmasks, dtype=np.uint64)
+        vb = vb_arr.ctypes.data_as(C.POINTER(C.c_uint64))
+    lib().orc_step_erd(_p32(cells), _p32(out), X, Y, Z, word0, word1, vb, max_rule_bits, C.byref(st))
+    return out, st
+
+
+def init_cells_erd(X, Y, Z, M, spawn, spawn_mode, density, seed) -> np.ndarray:
+    a = np.zeros(erd_words(X, Y, Z), dtype=np.uint32)
+    sx, sy, sz = spawn
+    lib().orc_init_cells_erd(_p32(a), X, Y, Z, M, sx, sy, sz, spawn_mode, float(density), seed)
+    return a
+
+
+def erd_hash(cells, X, Y, Z) -> int:
+    return int(lib().orc_erd_canonical_hash(_p32(cells), X, Y, Z))
+
+
+def run_rule_erd(X, Y, Z, M, flags, birth, survival, max_ticks, initial):
+    info = GridInfo()
+    per_tick = (TickStats * max_ticks)()
+    final = np.zeros_like(initial)
+    t = lib().orc_run_rule_erd(X, Y, Z, M, flags, birth, survival, max_ticks, _p32(initial), C.byref(info),
+                               per_tick, _p32(final))
+    return int(t), info, [per_tick[i] for i in range(min(t, max_ticks))], final
+
+
+def run_search_shape(grid, shape, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn, spawn_mode,
+                     density, seed, n_threads=8, legacy_death=0):
+    counts = (C.c_uint64 * 3)()
+    n = lib().orc_run_search_shape(grid, shape, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn,
+                                   spawn_mode, float(density), seed, n_threads, counts, legacy_death)
+    return int(n), (int(counts[0]), int(counts[1]), int(counts[2]))
 
 
 def step_render(cells, X, Y, Z, word0, word1):

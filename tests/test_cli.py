@@ -17,11 +17,11 @@ def test_help():
 
 def test_validation_errors_exit_1():
     assert run("--mode", "search", "--neighborhood", "X").returncode == 1
-    assert run("--mode", "search", "--neighborhood", "FW").returncode == 1
+    assert run("--mode", "search", "--neighborhood", "W").returncode == 1  # W alone selects no neighbours
     assert run("--mode", "search", "--search-workgroup-size", "7").returncode == 1
     assert run("--mode", "search", "--spawn-mode", "sphere").returncode == 1
     assert run("--mode", "search", "--grid-sweep", "30:20:5").returncode == 1
-    assert run("--mode", "search", "--shape", "ed").returncode == 1
+    assert run("--mode", "search", "--shape", "dodecahedron").returncode == 1
     assert run("--mode", "video").returncode == 1
     r = run("--mode", "search", "--neighborhood", "X")
     assert "--neighborhood must include at least one of F / E / C" in r.stderr
@@ -31,3 +31,8 @@ def test_parse_errors_exit_2():
     assert run("--grid").returncode == 2
     assert run("--grid", "abc", "--mode", "search").returncode == 2
     assert run("positional").returncode == 2
+
+
+def test_unknown_shape_exits_1():
+    out = subprocess.run([CLI, "--mode", "search", "--shape", "hexagon"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "Unknown --shape" in (out.stdout + out.stderr)

@@ -343,3 +343,54 @@ def test_render_init_is_unsalted_hash_with_float_threshold():
     a = O.init_cells_render(X, Y, Z, M, spawn, 0, 0.5)
     b = O.init_cells(X, Y, Z, M, spawn, 0, 0.5, 0)
     assert np.array_equal(a, b)
+
+
+# ---- ERD shape (searchSimulationERD.slang), pinned by the honeycomb's geometry ----
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 2), (5, 4, 3), (9, 7, 6), (13, 8, 5)])
+@pytest.mark.parametrize("flags", [1, 2, 3, 9, 10, 11])
+def test_erd_step_matches_geometric_model(dims, flags):
+    X, Y, Z = dims
+    rng = np.random.default_rng(X * 1000 + Y * 10 + Z + flags)
+    N = O.lib().orc_erd_neighbor_count(flags)
+    for M in (1, 4, 15):
+        cells = rng.integers(0, M + 1, size=(Z, Y, X)).astype(np.uint8)
+        birth, surv = int(rng.integers(1, 1 << N)), int(rng.integers(1, 1 << N))
+        w0 = ((M - 1) & 0xF) << 60 | birth
+        w1 = flags << 60 | surv
+        out, st = O.step_erd(NP.pack_linear(cells), X, Y, Z, w0, w1)
+        want, counts = NP.erd_step(cells, M, flags & 3, birth, surv, wrap=bool(flags & 8))
+        assert np.array_equal(NP.unpack_linear(out, X, Y, Z), want)
+        assert st.as_tuple()[:4] == counts
+
+
+def test_erd_neighbour_relation_is_symmetric_and_complete():
+    # a lone live cell with B = {1} births exactly its 12 / 4 / 16 neighbours,
+    # and every cell it births births it back (faces are shared)
+    X = Y = Z = 9
+    for flags, n in ((1, 12), (2, 4), (3, 16)):
+        for cy in (4, 5):
+            cells = np.zeros((Z, Y, X), np.uint8)
+            cells[4, cy, 4] = 1
+            out, st = O.step_erd(NP.pack_linear(cells), X, Y, Z, 0 << 60 | 1, flags << 60 | 0)
+            born = np.argwhere(NP.unpack_linear(out, X, Y, Z) == 1)
+            assert st.snap.numberOfCellsBorn == n == len(born)
+            for z, y, x in born:
+                back = np.zeros((Z, Y, X), np.uint8)
+                back[z, y, x] = 1
+                o2, _ = O.step_erd(NP.pack_linear(back), X, Y, Z, 0 << 60 | 1, flags << 60 | 0)
+                assert NP.unpack_linear(o2, X, Y, Z)[4, cy, 4] == 1
+
+
+def test_erd_init_and_hash():
+    X, Y, Z, M = 11, 9, 7, 3
+    a = O.init_cells_erd(X, Y, Z, M, (6, 5, 4), 0, 0.4, 17)
+    b = O.init_cells(X, Y, Z, M, (6, 5, 4), 0, 0.4, 17)
+    # same cells, linear vs row-padded layout; the same canonical hash
+    assert np.array_equal(NP.unpack_linear(a, X, Y, Z), NP.unpack(b, X, Y, Z))
+    assert O.erd_hash(a, X, Y, Z) == O.canonical_hash(b, X, Y, Z)
+
+
+def test_erd_search_counts_cover_the_rule_space():
+    n, (u, p, v) = O.run_search_shape(7, 1, 3, 2, 2, 2, 6, 3, 1, 0.5, 0, 8)
+    # F (12), E (4), FE (16) slots at K = 2
+    assert n == sum((N + N * (N - 1) // 2) ** 2 for N in (12, 4, 16)) == u + p + v
