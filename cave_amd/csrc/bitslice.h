@@ -155,6 +155,19 @@ CAVE_HD Bits<2> kind2(const RowSums &s)
 }
 
 // ---- rule ----
+// gfx950 three-input bit operations (v_bitop3_b32; truth table over S0 = 0xF0,
+// S1 = 0xCC, S2 = 0xAA): one instruction for a XOR feeding an AND, and for an
+// AND feeding an OR.
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c)
+{   // (a ^ b) & c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x28);
+}
+
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t b, uint32_t c)
+{   // (a & b) | c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xEA);
+}
+
 // Evaluates the born / survive tables for 32 cells at once: for every count v
 // that appears in either table, the minterm [N == v] is built from the five
 // count planes and OR-ed into the matching result.  `tables` = born | surv.
@@ -174,10 +187,14 @@ CAVE_HD void apply_rule(const Bits<5> &n, uint32_t born_lut, uint32_t surv_lut, 
     }
 }
 
-// Same tables for R rows at once: the scalar loop over the set counts (a
-// wavefront-uniform value) is shared by the rows.
+// Same tables for R rows at once.  The scalar loop over the counts present in
+// either table (a wavefront-uniform value) is shared by the rows; per count
+// and row the minterm [N == v] is a chain of five XOR-ANDs (the XOR applies
+// the count bit's polarity), and each table takes it with one AND-OR against
+// a uniform all-ones / zero mask -- seven instructions, no branches.
 template <int R>
-CAVE_HD void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_lut, uint32_t *tb, uint32_t *ts)
+__device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_lut, uint32_t *tb,
+                                                uint32_t *ts)
 {
 #pragma unroll
     for (int r = 0; r < R; r++) tb[r] = ts[r] = 0u;
@@ -192,11 +209,11 @@ CAVE_HD void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_
         for (int b = 0; b < 5; b++) inv[b] = ((v >> b) & 1u) ? 0u : ~0u;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            uint32_t mt = ~0u;
+            uint32_t mt = n[r].b[0] ^ inv[0];
 #pragma unroll
-            for (int b = 0; b < 5; b++) mt &= n[r].b[b] ^ inv[b];
-            tb[r] |= mt & bm;
-            ts[r] |= mt & sm;
+            for (int b = 1; b < 5; b++) mt = xor_and(n[r].b[b], inv[b], mt);
+            tb[r] = and_or(mt, bm, tb[r]);
+            ts[r] = and_or(mt, sm, ts[r]);
         }
     }
 }
