@@ -79,7 +79,7 @@ def run_step_bench(steps: int, warmup: int):
     h = cv.Step((GRID, GRID, GRID), 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv,
                 spawn=(GRID, GRID, GRID), spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42)
     if warmup:
-        h.run(warmup, stats=False)
+        h.run(warmup, stats=True)
     t0 = time.perf_counter()
     _, dev_ms = h.run(steps, stats=True)
     wall = time.perf_counter() - t0

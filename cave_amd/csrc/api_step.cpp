@@ -39,6 +39,7 @@ struct CaveStep {
     uint32_t *cur = nullptr, *nxt = nullptr;
     StatPart *partials = nullptr;   // [chunk ticks][nparts]
     CaveTickStats *dstats = nullptr; // [chunk ticks]
+    CaveTickStats *hstats = nullptr; // [chunk ticks], pinned host staging for the per-tick copy
     uint32_t chunk = 0;
     // instantiated tick loops: one graph per (ticks, which buffer is current)
     struct TickGraph {
@@ -122,6 +123,7 @@ void release(CaveStep *h)
     if (h->nxt) (void)hipFree(h->nxt);
     if (h->partials) (void)hipFree(h->partials);
     if (h->dstats) (void)hipFree(h->dstats);
+    if (h->hstats) (void)hipHostFree(h->hstats);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -218,6 +220,10 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     if (const char *env = std::getenv("CAVE_STEP_GRAPHS")) h->use_graphs = env[0] == '1';
     CK(hipMalloc(&h->partials, sizeof(StatPart) * (size_t)h->nparts * h->chunk));
     CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
+    // Per-tick stats go device -> pinned staging -> caller: a device-to-host
+    // copy straight into a caller's pageable buffer of this size costs the
+    // runtime milliseconds per call.
+    CK(hipHostMalloc((void **)&h->hstats, sizeof(CaveTickStats) * h->chunk, hipHostMallocDefault));
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
         CK(hipStreamSynchronize(h->stream));
         rc = cave_step_upload(h, spec->initial_cells, dev_words(h));
@@ -259,9 +265,9 @@ int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *
         int rc = launch_reduce_ticks(h->partials, h->nparts, n, h->dstats, h->stream);
         if (rc) return rc;
         if (per_tick)
-            CAVE_HIP(hipMemcpyAsync(per_tick + done, h->dstats, sizeof(CaveTickStats) * n, hipMemcpyDeviceToHost,
-                                    h->stream));
+            CAVE_HIP(hipMemcpyAsync(h->hstats, h->dstats, sizeof(CaveTickStats) * n, hipMemcpyDeviceToHost, h->stream));
         CAVE_HIP(hipStreamSynchronize(h->stream));
+        if (per_tick) std::memcpy(per_tick + done, h->hstats, sizeof(CaveTickStats) * n);
         float ms = 0.f;
         CAVE_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         total_ms += ms;

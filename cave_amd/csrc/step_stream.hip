@@ -126,19 +126,20 @@ struct StreamLane {
         if (!PLANAR) transpose4x4(P[0], P[1], P[2], P[3]);
     }
 
-    // One z step: consume plane zi (ci.w), refill ci.w with plane zi+2 (the
-    // step after next runs on the same register set: two planes in flight),
-    // finalize plane zi-1.
-    __device__ __forceinline__ void step(int zi, Carry<RPL> &ci, Carry<RPL> &co)
+    // One z step of a walk in direction dz (+1 or -1): consume plane zi
+    // (ci.w), load plane zi+dz into co.w when `more` (the next step consumes
+    // it), finalize plane zi-dz when `fin`.  The z stencil is symmetric, so a
+    // descending walk computes the same planes.
+    __device__ __forceinline__ void step(int zi, int dz, bool more, bool fin, Carry<RPL> &ci, Carry<RPL> &co)
     {
         const RuleConsts &rc = a.rc;
         RowSums rs[RPL + 1];
         uint4 wcur[RPL + 1];
 #pragma unroll
         for (int r = 0; r <= RPL; r++) wcur[r] = ci.w[r];
-        if (zi < z1) {
+        if (more) {
 #pragma unroll
-            for (int r = 0; r <= RPL; r++) co.w[r] = load(r, zi + 1);
+            for (int r = 0; r <= RPL; r++) co.w[r] = load(r, zi + dz);
         }
 #pragma unroll
         for (int r = 0; r <= RPL; r++) {
@@ -197,9 +198,9 @@ struct StreamLane {
             co.S1p[r] = S1c[r];
         }
 
-        // finalize plane z = zi - 1 (uniform over the wave)
-        const int z = zi - 1;
-        if (z >= z0) {
+        // finalize plane z = zi - dz (uniform over the wave)
+        const int z = zi - dz;
+        if (fin) {
             Bits<5> n[RPL];
             uint32_t tb[RPL], ts[RPL];
 #pragma unroll
@@ -236,13 +237,22 @@ constexpr uint32_t kOutOfRange = 0x80000000u;  // any row offset this large is o
 constexpr uint32_t kBufferDword3 = 0x00020000u;  // raw buffer resource word 3 on gfx9-family parts
 
 template <uint32_t FLAGS, int RPL, bool PLANAR>
-__global__ void __launch_bounds__(256) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <= 2 ? 4 : 1))) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
                                                      StreamArgs a, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
 {
     __shared__ int32_t red[10 * 4];
     if (done && *done) return;  // the manager already ended this simulation
-    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8).  When
+    // the grid is a multiple of 8 blocks, XCD x takes the contiguous logical
+    // blocks x*nb/8 .. (x+1)*nb/8-1, so the y and z halos a wave shares with
+    // its neighbours mostly sit in the same XCD's L2.
+    const uint32_t nb = gridDim.x;
+    const uint32_t lblock = (nb & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3);
+    // readfirstlane: the wave index is uniform, and saying so keeps the tile,
+    // z range and plane offsets in scalar registers
+    const uint32_t gwave =
+        lblock * (blockDim.x >> 6) + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     StatAcc acc;
 
     if (gwave < a.total_waves) {
@@ -280,18 +290,26 @@ __global__ void __launch_bounds__(256) k_step_stream(const uint4 *__restrict__ i
             for (int k = 0; k < 4; k++) c0.S1p[r].b[k] = 0u;
             for (int k = 0; k < 5; k++) c0.Q[r].b[k] = 0u;
         }
+        // Odd z tiles walk downwards, so the two tiles on either side of a
+        // tile boundary read the boundary planes at about the same time (both
+        // at their first or both at their last steps) and the second read
+        // hits L2.
+        const int dz = (ztile & 1u) ? -1 : 1;
+        const int zs = dz > 0 ? L.z0 : L.z1 - 1;  // first owned plane of the walk
+        const int n = L.z1 - L.z0;                 // owned planes
 #pragma unroll
         for (int r = 0; r <= RPL; r++) {
-            c0.w[r] = L.load(r, L.z0 - 1);
+            c0.w[r] = L.load(r, zs - dz);
         }
 
-        // steps zi = z0-1 .. z1, two per iteration with alternating register sets
-        int zi = L.z0 - 1;
-        for (; zi + 1 <= L.z1; zi += 2) {
-            L.step(zi, c0, c1);
-            L.step(zi + 1, c1, c0);
+        // steps t = 0 .. n+1 on planes zs + dz*(t-1), two per iteration with
+        // alternating register sets
+        int t = 0;
+        for (; t + 1 <= n + 1; t += 2) {
+            L.step(zs + dz * (t - 1), dz, t <= n, t >= 2, c0, c1);
+            L.step(zs + dz * t, dz, t + 1 <= n, t + 1 >= 2, c1, c0);
         }
-        if (zi <= L.z1) L.step(zi, c0, c1);
+        if (t <= n + 1) L.step(zs + dz * (t - 1), dz, t <= n, t >= 2, c0, c1);
 
         acc.surv = L.surv;
         acc.born = L.born;
