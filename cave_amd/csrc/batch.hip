@@ -141,12 +141,12 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
                              shift_right_neighbor_nat(av[d], nv[d] & nm));
         const Bits<4> S0c = add3<4>(kind1<FLAGS>(rs[0]), kind0<FLAGS>(rs[1]), kind1<FLAGS>(rs[2]));
         const Bits<4> S1c = add3<4>(kind2<FLAGS>(rs[0]), kind1<FLAGS>(rs[1]), kind2<FLAGS>(rs[2]));
-        co.Q = add3<5>(ci.S1p, S0c, Bits<1>{{0u}});
+        co.Q = add2<5>(ci.S1p, S0c);
         co.S1p = S1c;
         co.A = rs[1].a;
         const int z = zi - 1;
         if (z >= z0) {
-            const Bits<5> n = add3<5>(ci.Q, S1c, Bits<1>{{0u}});
+            const Bits<5> n = add2<5>(ci.Q, S1c);
             uint32_t tb, ts;
             apply_rule_rows<1>(&n, rc.born_lut, rc.surv_lut, &tb, &ts);
             uint32_t *P = S + (uint32_t)w + g.W * ((uint32_t)y + g.Y * (uint32_t)z);

@@ -95,6 +95,24 @@ CAVE_HD Bits<W> add3(const Bits<NA> &a, const Bits<NB> &b, const Bits<NC> &c)
     return r;
 }
 
+// a + b, W output planes, as a ripple-carry adder with one carry per column
+// (sum = a ^ b ^ k, carry = majority(a, b, k): one three-input bit operation
+// each).  add3 with a zero third operand costs two half adders per column.
+template <int W, int NA, int NB>
+CAVE_HD Bits<W> add2(const Bits<NA> &a, const Bits<NB> &b)
+{
+    Bits<W> r;
+    uint32_t k = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t ai = i < NA ? a.b[i < NA ? i : 0] : 0u;
+        const uint32_t bi = i < NB ? b.b[i < NB ? i : 0] : 0u;
+        r.b[i] = ai ^ bi ^ k;
+        k = (ai & bi) | (k & (ai ^ bi));
+    }
+    return r;
+}
+
 // ---- per-row x-sums ----
 // From a row's live mask A and its x-neighbour masks L/R:
 //   A  : the cell itself          (1 plane)

@@ -75,6 +75,13 @@ __device__ __forceinline__ void mask_extent(uint32_t m, int &lo, int &hi)
     }
 }
 
+constexpr uint32_t kOutOfRange = 0x80000000u;  // any row offset this large is outside every buffer
+// Plane offset of a load that must read nothing: buffers stay below 2^30 bytes
+// (stream_supported), and a row offset (< 2^30, or kOutOfRange) plus 2^30
+// neither wraps nor lands inside one.
+constexpr uint32_t kNoPlane = 0x40000000u;
+constexpr uint32_t kBufferDword3 = 0x00020000u;  // raw buffer resource word 3 on gfx9-family parts
+
 template <int RPL>
 struct Carry {
     uint4 w[RPL + 1];      // raw groups of the plane the step receiving this set consumes
@@ -104,10 +111,15 @@ struct StreamLane {
     uint32_t yany[RPL];
     int zmin = 0x7fffffff, zmax = -1;
 
+    __device__ __forceinline__ uint4 load_at(int r, uint32_t zoff) const
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rin, (int)(roff[r] + zoff), 0, 0);
+        return make_uint4(v[0], v[1], v[2], v[3]);
+    }
+
     __device__ __forceinline__ uint4 load(int r, int z) const
     {   // z = -1 wraps to a huge offset and z = Z lands at the buffer end: both read zeros
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rin, (int)(roff[r] + (uint32_t)z * plane_bytes), 0, 0);
-        return make_uint4(v[0], v[1], v[2], v[3]);
+        return load_at(r, (uint32_t)z * plane_bytes);
     }
 
     __device__ __forceinline__ void store(int r, int z, const uint32_t P[4]) const
@@ -137,10 +149,12 @@ struct StreamLane {
         uint4 wcur[RPL + 1];
 #pragma unroll
         for (int r = 0; r <= RPL; r++) wcur[r] = ci.w[r];
-        if (more) {
+        // Unconditional: past the walk's last plane the load gets an offset
+        // the range check rejects (no memory access, zeros, never consumed).
+        // A conditional load would be waited on at once.
+        const uint32_t zoff = more ? (uint32_t)(zi + dz) * plane_bytes : kNoPlane;
 #pragma unroll
-            for (int r = 0; r <= RPL; r++) co.w[r] = load(r, zi + dz);
-        }
+        for (int r = 0; r <= RPL; r++) co.w[r] = load_at(r, zoff);
 #pragma unroll
         for (int r = 0; r <= RPL; r++) {
             uint32_t P[4];
@@ -194,7 +208,7 @@ struct StreamLane {
             const Bits<2> d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
             const Bits<4> S0c = add3<4>(u1, kind0<FLAGS>(rs[r]), d1);
             S1c[r] = add3<4>(u2, kind1<FLAGS>(rs[r]), d2);
-            co.Q[r] = add3<5>(ci.S1p[r], S0c, Bits<1>{{0u}});
+            co.Q[r] = add2<5>(ci.S1p[r], S0c);
             co.S1p[r] = S1c[r];
         }
 
@@ -204,7 +218,7 @@ struct StreamLane {
             Bits<5> n[RPL];
             uint32_t tb[RPL], ts[RPL];
 #pragma unroll
-            for (int r = 0; r < RPL; r++) n[r] = add3<5>(ci.Q[r], S1c[r], Bits<1>{{0u}});
+            for (int r = 0; r < RPL; r++) n[r] = add2<5>(ci.Q[r], S1c[r]);
             apply_rule_rows<RPL>(n, rc.born_lut, rc.surv_lut, tb, ts);
             uint32_t any = 0u;
 #pragma unroll
@@ -233,8 +247,6 @@ struct StreamLane {
     }
 };
 
-constexpr uint32_t kOutOfRange = 0x80000000u;  // any row offset this large is outside every buffer
-constexpr uint32_t kBufferDword3 = 0x00020000u;  // raw buffer resource word 3 on gfx9-family parts
 
 template <uint32_t FLAGS, int RPL, bool PLANAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <= 2 ? 4 : 1))) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
