@@ -107,14 +107,25 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
     const uint32_t vmask = (uint32_t)w + 1u < g.W ? ~0u : g.last_mask;
     const uint32_t pm = w > 0 ? ~0u : 0u;
     const uint32_t nm = (uint32_t)w + 1u < g.W ? ~0u : 0u;
-    // live word of row (y-1) at plane zi: acur + w + W*(y + (Y+2)*(zi+1))
-    const uint32_t *A = S + acur + w + g.W * (uint32_t)y;
     uint32_t colmask = 0u;
     int zmin = 0x7fffffff, zmax = -1;
+    // Running word offsets for the step at plane zi (the first step has
+    // zi = z0 - 1), advanced by one plane per step instead of recomputed with
+    // multiplies:
+    //   aoff: live word of row y-1 at plane zi:      acur + w + W*(y + (Y+2)*(zi+1))
+    //   poff: state word of row y at plane zi-1:     w + W*(y + Y*(zi-1))
+    //   noff: next live word of row y at plane zi-1: anext + w + W*((y+1) + (Y+2)*zi)
+    // poff and noff wrap below plane 0 on the first steps; they are used only
+    // once plane zi-1 is one of the item's planes.
+    const uint32_t wy = (uint32_t)w + g.W * (uint32_t)y;
+    const uint32_t pstride = g.W * g.Y;
+    uint32_t aoff = acur + wy + g.aslab * (uint32_t)z0;
+    uint32_t poff = wy + pstride * (uint32_t)(z0 - 2);
+    uint32_t noff = anext + wy + g.W + g.aslab * (uint32_t)(z0 - 1);
 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
         RowSums rs[3];
-        const uint32_t *a0 = A + g.aslab * (uint32_t)(zi + 1);
+        const uint32_t *a0 = S + aoff;
         // planes zlo-1, zlo, zhi-1, zhi are written or read by a neighbouring part
         const bool shr = COH && (zi <= (int)g.zlo || zi + 1 >= (int)g.zhi);
         uint32_t av[3], pv[3], nv[3];
@@ -149,7 +160,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
             const Bits<5> n = add2<5>(ci.Q, S1c);
             uint32_t tb, ts;
             apply_rule_rows<1>(&n, rc.born_lut, rc.surv_lut, &tb, &ts);
-            uint32_t *P = S + (uint32_t)w + g.W * ((uint32_t)y + g.Y * (uint32_t)z);
+            uint32_t *P = S + poff;
             uint32_t pl[4] = {P[0], P[g.pplane], P[2u * g.pplane], P[3u * g.pplane]};
             const uint32_t NZ = nonzero_mask(pl);
             const uint32_t Al = ci.A;
@@ -163,14 +174,16 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
             P[g.pplane] = pl[1];
             P[2u * g.pplane] = pl[2];
             P[3u * g.pplane] = pl[3];
-            st_live<COH>(S + anext + (uint32_t)w + g.W * ((uint32_t)(y + 1) + (g.Y + 2u) * (uint32_t)(z + 1)), set_m,
-                         COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
+            st_live<COH>(S + noff, set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
             colmask |= set_m;
             if (set_m) {
                 zmin = min(zmin, z);
                 zmax = max(zmax, z);
             }
         }
+        aoff += g.aslab;
+        poff += pstride;
+        noff += g.aslab;
     };
 
     BCarry c0, c1;
