@@ -75,8 +75,8 @@ struct BCarry {
 // device-scope atomic loads / stores: on gfx950 these are `sc1` accesses that
 // are coherent across the XCDs' L2 caches without a cache-wide write-back or
 // invalidate, so a split grid does not disturb the other blocks' cached cells.
-template <bool COH>
-__device__ __forceinline__ uint32_t ld_live(const uint32_t *p, bool shared)
+template <bool COH, typename T>
+__device__ __forceinline__ uint32_t ld_live(const T *p, bool shared)
 {
     if constexpr (COH) {
         if (shared) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -84,8 +84,8 @@ __device__ __forceinline__ uint32_t ld_live(const uint32_t *p, bool shared)
     return *p;
 }
 
-template <bool COH>
-__device__ __forceinline__ void st_live(uint32_t *p, uint32_t v, bool shared)
+template <bool COH, typename T>
+__device__ __forceinline__ void st_live(T *p, uint32_t v, bool shared)
 {
     if constexpr (COH) {
         if (shared) {
@@ -96,12 +96,38 @@ __device__ __forceinline__ void st_live(uint32_t *p, uint32_t v, bool shared)
     *p = v;
 }
 
+// A job's storage words, addressed by 32-bit word offsets.
+//   * Words<false>: the block's LDS (a pointer the compiler resolves to LDS,
+//     so accesses are ds_* instructions with 32-bit addresses);
+//   * Words<true>: global memory (the block's arena region or the job's own
+//     region) as an address_space(1) base plus a 32-bit byte offset, which
+//     the compiler turns into global accesses with the base in scalar
+//     registers instead of 64-bit address arithmetic per access.  Regions
+//     stay far below 2^30 words, so the byte offset fits 32 bits.
+typedef __attribute__((address_space(1))) uint32_t gword;
+typedef __attribute__((address_space(1))) uint8_t gbyte;
+
+template <bool GLOBAL>
+struct Words {
+    uint32_t *base;
+    __device__ __forceinline__ uint32_t *at(uint32_t off) const { return base + off; }
+};
+
+template <>
+struct Words<true> {
+    gword *base;
+    __device__ __forceinline__ gword *at(uint32_t off) const
+    {
+        return reinterpret_cast<gword *>(reinterpret_cast<gbyte *>(base) + (off << 2));
+    }
+};
+
 // One item: column (w, y), planes [z0, z1).  A_cur / A_next / P are offsets
 // into S.  Loads of the x-neighbour words of a row always hit valid storage
 // (the previous / next row or a guard row); words outside the row are masked.
 // COH: the job is split over several blocks (boundary planes are shared).
-template <uint32_t FLAGS, bool COH>
-__device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext,
+template <uint32_t FLAGS, bool COH, bool GLOBAL>
+__device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                                          const RuleConsts &rc, int w, int y, int z0, int z1, StatAcc &acc)
 {
     const uint32_t vmask = (uint32_t)w + 1u < g.W ? ~0u : g.last_mask;
@@ -125,14 +151,13 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
         RowSums rs[3];
-        const uint32_t *a0 = S + aoff;
         // planes zlo-1, zlo, zhi-1, zhi are written or read by a neighbouring part
         const bool shr = COH && (zi <= (int)g.zlo || zi + 1 >= (int)g.zhi);
         uint32_t av[3], pv[3], nv[3];
         if (shr) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const uint32_t *p = a0 + d * g.W;
+                const auto *p = S.at(aoff + (uint32_t)d * g.W);
                 av[d] = ld_live<COH>(p, true);
                 pv[d] = ld_live<COH>(p - 1, true);
                 nv[d] = ld_live<COH>(p + 1, true);
@@ -140,7 +165,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
         } else {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const uint32_t *p = a0 + d * g.W;
+                const auto *p = S.at(aoff + (uint32_t)d * g.W);
                 av[d] = p[0];
                 pv[d] = p[-1];
                 nv[d] = p[1];
@@ -160,8 +185,9 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
             const Bits<5> n = add2<5>(ci.Q, S1c);
             uint32_t tb, ts;
             apply_rule_rows<1>(&n, rc.born_lut, rc.surv_lut, &tb, &ts);
-            uint32_t *P = S + poff;
-            uint32_t pl[4] = {P[0], P[g.pplane], P[2u * g.pplane], P[3u * g.pplane]};
+            auto *P0 = S.at(poff), *P1 = S.at(poff + g.pplane), *P2 = S.at(poff + 2u * g.pplane),
+                 *P3 = S.at(poff + 3u * g.pplane);
+            uint32_t pl[4] = {*P0, *P1, *P2, *P3};
             const uint32_t NZ = nonzero_mask(pl);
             const uint32_t Al = ci.A;
             const uint32_t set_m = vmask & ((Al & ts) | (~NZ & tb));
@@ -170,11 +196,11 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
             acc.born += __popc(set_m ^ surv);
             acc.dying += __popc(NZ & ~surv);
             update_planes(pl, set_m, NZ, rc.mk);
-            P[0] = pl[0];
-            P[g.pplane] = pl[1];
-            P[2u * g.pplane] = pl[2];
-            P[3u * g.pplane] = pl[3];
-            st_live<COH>(S + noff, set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
+            *P0 = pl[0];
+            *P1 = pl[1];
+            *P2 = pl[2];
+            *P3 = pl[3];
+            st_live<COH>(S.at(noff), set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
             colmask |= set_m;
             if (set_m) {
                 zmin = min(zmin, z);
@@ -206,9 +232,9 @@ __device__ __forceinline__ void run_item(const JobGeo &g, uint32_t *S, uint32_t 
     }
 }
 
-template <uint32_t FLAGS, bool COH>
-__device__ void tick_items(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext, const RuleConsts &rc,
-                           StatAcc &acc)
+template <uint32_t FLAGS, bool COH, bool GLOBAL>
+__device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
+                           const RuleConsts &rc, StatAcc &acc)
 {
     for (uint32_t it = threadIdx.x; it < g.nitems; it += blockDim.x) {
         const int w = (int)(it % g.W);
@@ -217,23 +243,23 @@ __device__ void tick_items(const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t
         const uint32_t zc = g.zc0 + rest / g.Y;
         const int z0 = (int)(zc * g.tzc);
         const int z1 = min(z0 + (int)g.tzc, (int)g.Z);
-        run_item<FLAGS, COH>(g, S, acur, anext, rc, w, y, z0, z1, acc);
+        run_item<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
     }
 }
 
-template <bool COH>
-__device__ void run_tick(uint32_t flags, const JobGeo &g, uint32_t *S, uint32_t acur, uint32_t anext,
+template <bool COH, bool GLOBAL>
+__device__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                          const RuleConsts &rc, StatAcc &acc)
 {
     switch (flags & 7u) {
-    case 1: tick_items<1, COH>(g, S, acur, anext, rc, acc); break;
-    case 2: tick_items<2, COH>(g, S, acur, anext, rc, acc); break;
-    case 3: tick_items<3, COH>(g, S, acur, anext, rc, acc); break;
-    case 4: tick_items<4, COH>(g, S, acur, anext, rc, acc); break;
-    case 5: tick_items<5, COH>(g, S, acur, anext, rc, acc); break;
-    case 6: tick_items<6, COH>(g, S, acur, anext, rc, acc); break;
-    case 7: tick_items<7, COH>(g, S, acur, anext, rc, acc); break;
-    default: tick_items<0, COH>(g, S, acur, anext, rc, acc); break;
+    case 1: tick_items<1, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 2: tick_items<2, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 3: tick_items<3, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 4: tick_items<4, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 5: tick_items<5, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 6: tick_items<6, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 7: tick_items<7, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    default: tick_items<0, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
     }
 }
 
@@ -241,16 +267,16 @@ __device__ void run_tick(uint32_t flags, const JobGeo &g, uint32_t *S, uint32_t 
 // for this part's planes [zlo, zhi).  A part owns the live-plane slabs of its
 // planes (the first part also the z = -1 guard slab, the last part the z = Z
 // one): it clears them in both live planes, then writes its live bits.
-template <bool COH>
-__device__ void init_cells(const DevJob &j, const JobGeo &g, uint32_t *S, uint32_t acur)
+template <bool COH, bool GLOBAL>
+__device__ void init_cells(const DevJob &j, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur)
 {
     const uint32_t parts = max(1u, j.parts);
     const uint32_t s0 = j.part == 0u ? 0u : g.zlo + 1u;
     const uint32_t s1 = j.part + 1u >= parts ? g.Z + 2u : g.zhi + 1u;
     const uint32_t nclr = (s1 > s0 ? s1 - s0 : 0u) * g.aslab;
     for (uint32_t i = threadIdx.x; i < nclr; i += blockDim.x) {
-        st_live<COH>(S + 4u * g.pplane + s0 * g.aslab + i, 0u, true);
-        st_live<COH>(S + 4u * g.pplane + g.aplane + s0 * g.aslab + i, 0u, true);
+        st_live<COH>(S.at(4u * g.pplane + s0 * g.aslab + i), 0u, true);
+        st_live<COH>(S.at(4u * g.pplane + g.aplane + s0 * g.aslab + i), 0u, true);
     }
     __syncthreads();
     const uint32_t live = j.M <= 15u ? ~0u : 0u;
@@ -269,11 +295,11 @@ __device__ void init_cells(const DevJob &j, const JobGeo &g, uint32_t *S, uint32
             for (uint32_t x = xa; x < xb; x++)
                 if (init_alive(j.init, x, y, z)) bits |= 1u << (x - 32u * w);
         }
-        S[idx] = (j.M & 1u) ? bits : 0u;
-        S[g.pplane + idx] = (j.M & 2u) ? bits : 0u;
-        S[2u * g.pplane + idx] = (j.M & 4u) ? bits : 0u;
-        S[3u * g.pplane + idx] = (j.M & 8u) ? bits : 0u;
-        st_live<COH>(S + acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u)), bits & live, true);
+        *S.at(idx) = (j.M & 1u) ? bits : 0u;
+        *S.at(g.pplane + idx) = (j.M & 2u) ? bits : 0u;
+        *S.at(2u * g.pplane + idx) = (j.M & 4u) ? bits : 0u;
+        *S.at(3u * g.pplane + idx) = (j.M & 8u) ? bits : 0u;
+        st_live<COH>(S.at(acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u))), bits & live, true);
     }
 }
 
@@ -437,9 +463,11 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     const bool multi = j.parts > 1u;
     uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
     uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;
-    if (j.tier == kTierLds) init_cells<false>(j, g, lds, acur);
-    else if (multi) init_cells<true>(j, g, S, acur);
-    else init_cells<false>(j, g, S, acur);
+    const Words<false> SL{lds};
+    const Words<true> SG{(gword *)S};  // generic -> global address space
+    if (j.tier == kTierLds) init_cells<false>(j, g, SL, acur);
+    else if (multi) init_cells<true>(j, g, SG, acur);
+    else init_cells<false>(j, g, SG, acur);
     CaveGridInfo info{};
     CaveTickStats last{};
     const uint32_t cells = j.X * j.Y * j.Z;
@@ -451,9 +479,9 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     uint32_t ticks_run = 0;  // the same in every thread and every part
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
         StatAcc acc;
-        if (j.tier == kTierLds) run_tick<false>(j.flags, g, lds, acur, anext, rc, acc);
-        else if (multi) run_tick<true>(j.flags, g, S, acur, anext, rc, acc);
-        else run_tick<false>(j.flags, g, S, acur, anext, rc, acc);
+        if (j.tier == kTierLds) run_tick<false>(j.flags, g, SL, acur, anext, rc, acc);
+        else if (multi) run_tick<true>(j.flags, g, SG, acur, anext, rc, acc);
+        else run_tick<false>(j.flags, g, SG, acur, anext, rc, acc);
         ticks_run++;
         uint32_t tot[3];
         int32_t bmin[3], bmax[3];
