@@ -175,9 +175,13 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
         for (int d = 0; d < 3; d++)
             rs[d] = row_sums(av[d], shift_left_neighbor_nat(av[d], pv[d] & pm),
                              shift_right_neighbor_nat(av[d], nv[d] & nm));
-        const Bits<4> S0c = add3<4>(kind1<FLAGS>(rs[0]), kind0<FLAGS>(rs[1]), kind1<FLAGS>(rs[2]));
         const Bits<4> S1c = add3<4>(kind2<FLAGS>(rs[0]), kind1<FLAGS>(rs[1]), kind2<FLAGS>(rs[2]));
-        co.Q = add2<5>(ci.S1p, S0c);
+        if constexpr (kBoxSum<FLAGS>) {
+            co.Q = add2<5>(ci.S1p, S1c);
+        } else {
+            const Bits<4> S0c = add3<4>(kind1<FLAGS>(rs[0]), kind0<FLAGS>(rs[1]), kind1<FLAGS>(rs[2]));
+            co.Q = add2<5>(ci.S1p, S0c);
+        }
         co.S1p = S1c;
         co.A = rs[1].a;
         const int z = zi - 1;
@@ -459,7 +463,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
 {
     const uint64_t t_start = wall_clock64();
     const JobGeo g = job_geo(j);
-    const RuleConsts rc = rule_consts(j.M, j.born_lut, j.surv_lut);
+    const RuleConsts rc = rule_consts_for(j.flags, j.M, j.born_lut, j.surv_lut);
     const bool multi = j.parts > 1u;
     uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
     uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;

@@ -276,4 +276,18 @@ CAVE_HD RuleConsts rule_consts(uint32_t M, uint32_t born_lut, uint32_t surv_lut)
     return c;
 }
 
+// FEV (all 26 neighbours): the count is the 3x3x3 box sum B of live cells
+// minus the cell itself, and B = S1(z-1) + S1(z) + S1(z+1) (with F, E and C
+// all set, k0 = k1 - A and k2 = k1, so S0 = S1 - A at the centre row).  The
+// kernels then sum S1 only and apply the rule to B: birth applies to dead
+// cells (not live, so N = B) and survival to live cells (N = B - 1), i.e. the
+// survival table shifted up by one count (B <= 27 fits the 32-bit table).
+template <uint32_t FLAGS>
+inline constexpr bool kBoxSum = (FLAGS & 7u) == 7u;
+
+CAVE_HD RuleConsts rule_consts_for(uint32_t flags, uint32_t M, uint32_t born_lut, uint32_t surv_lut)
+{
+    return rule_consts(M, born_lut, (flags & 7u) == 7u ? surv_lut << 1 : surv_lut);
+}
+
 }  // namespace cave

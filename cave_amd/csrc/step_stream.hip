@@ -206,9 +206,13 @@ struct StreamLane {
             const Bits<2> u2 = r == 0 ? k2_up : kind2<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
             const Bits<2> d1 = r == RPL - 1 ? k1_dn : kind1<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
             const Bits<2> d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
-            const Bits<4> S0c = add3<4>(u1, kind0<FLAGS>(rs[r]), d1);
             S1c[r] = add3<4>(u2, kind1<FLAGS>(rs[r]), d2);
-            co.Q[r] = add2<5>(ci.S1p[r], S0c);
+            if constexpr (kBoxSum<FLAGS>) {
+                co.Q[r] = add2<5>(ci.S1p[r], S1c[r]);
+            } else {
+                const Bits<4> S0c = add3<4>(u1, kind0<FLAGS>(rs[r]), d1);
+                co.Q[r] = add2<5>(ci.S1p[r], S0c);
+            }
             co.S1p[r] = S1c[r];
         }
 
@@ -527,7 +531,7 @@ int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
     a.total_waves = p.ytiles * p.ztiles;
     const uint32_t rem = g.X % 32u;
     a.last_mask = rem == 0 ? ~0u : (g.planar ? ((1u << rem) - 1u) : il_valid_mask(rem));
-    a.rc = rule_consts(rule.M, rule.born_lut, rule.surv_lut);
+    a.rc = rule_consts_for(rule.flags, rule.M, rule.born_lut, rule.surv_lut);
     const uint4 *i4 = reinterpret_cast<const uint4 *>(in);
     uint4 *o4 = reinterpret_cast<uint4 *>(out);
     if (g.planar) launch_layout<true>(rule.flags, i4, o4, a, p, partials, done, st);

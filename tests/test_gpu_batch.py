@@ -182,3 +182,17 @@ def test_empty_birth_or_survival_sets(engine):
                    spawn_mode=cv.SPAWN_RANDOM, density=0.4, seed=7, sim_flags=cv.SIM_FINAL_HASH)
             for M, fl, b, s in [(1, 1, 0, 0), (1, 1, 0, 0x3F), (3, 3, 0, 0), (2, 7, 0x3, 0), (5, 1, 0, 0x6)]]
     check(sims, engine)
+
+
+@pytest.mark.parametrize("engine", [cv.ENGINE_PERSISTENT, cv.ENGINE_PLAIN])
+def test_fev_top_counts(engine):
+    # nearly full grids: interior live cells see 26 live neighbours and most
+    # dead cells 26 as well; rules that fire only at counts 25 / 26 (mask bit
+    # b = count b+1), on LDS-tier (40x20x12, 49^3) and global-tier grids
+    sims = []
+    for i, dims in enumerate([(40, 20, 12), (49, 49, 49), (70, 40, 30)]):
+        for birth, surv in [(1 << 25, 1 << 25), (3 << 24, 3 << 24), (0x3FFFFFF, 1 << 25)]:
+            sims.append(cv.Sim(dims=dims, M=1 + 7 * i, flags=7, ticks=6, birth=birth, survival=surv, spawn=dims,
+                               spawn_mode=cv.SPAWN_RANDOM, density=0.97, seed=11 + i,
+                               sim_flags=cv.SIM_FINAL_HASH | cv.SIM_NO_EARLY_EXIT))
+    check(sims, engine)

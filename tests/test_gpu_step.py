@@ -150,6 +150,32 @@ def test_multi_tick_run_matches_single_ticks(kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
+@pytest.mark.parametrize("M", [1, 7, 15])
+def test_fev_top_counts(kernel, M):
+    """FEV counts up to 26: a fully live grid with isolated dead cells, so
+    interior live cells see 26 live neighbours and most dead cells 26 too,
+    under rules that fire only at the top counts (mask bit b = count b+1)."""
+    import np_model as NP
+
+    X, Y, Z = 70, 12, 10
+    rng = np.random.default_rng(M)
+    dense = np.full((Z, Y, X), M, dtype=np.uint8)
+    dense[rng.random((Z, Y, X)) < 0.03] = 0
+    packed = NP.pack(dense)
+    for birth, surv in [(1 << 25, 1 << 25), (3 << 24, 3 << 24), (0x3FFFFFF, 1 << 25)]:
+        w0 = ((M - 1) & 0xF) << 60 | birth
+        w1 = 7 << 60 | surv
+        print(f"FEV top counts M={M} B={birth:#x} S={surv:#x} kernel={KNAME[kernel]}", flush=True)
+        with cv.Step((X, Y, Z), M, 7, birth, surv, initial_cells=packed, kernel=kernel) as h:
+            ref = packed
+            for t in range(4):
+                ref, st_ref = O.step(ref, X, Y, Z, w0, w1)
+                (st,), _ = h.run(1)
+                assert st.as_tuple() == st_ref.as_tuple(), f"tick {t}"
+                assert np.array_equal(h.cells(), ref), f"tick {t}"
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
 def test_garbage_row_padding_is_ignored(kernel):
     # the reference never reads padding nibbles as neighbours; live-valued
     # garbage in them must not change any result
