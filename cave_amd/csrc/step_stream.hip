@@ -92,7 +92,9 @@ struct Carry {
     Bits<5> Q[RPL];        // S1 of the plane before it + S0 of the last input plane
 };
 
-template <uint32_t FLAGS, int RPL, bool PLANAR>
+// FULLROW: an x-row takes a whole 16-lane DPP row (16 lanes per row), so the
+// DPP moves' zero fill at the row ends already masks the x-neighbours.
+template <uint32_t FLAGS, int RPL, bool PLANAR, bool FULLROW>
 struct StreamLane {
     // Buffer resources over the two cell buffers: a 128-bit access whose byte
     // offset is at or beyond the buffer size reads zeros / is dropped by the
@@ -159,7 +161,7 @@ struct StreamLane {
         for (int r = 0; r <= RPL; r++) {
             uint32_t P[4];
             planes_of(wcur[r], P);
-            const uint32_t A = live_mask(P, rc.nm, rc.en);
+            const uint32_t A = live_mask_v3(P, rc.nm, rc.en);
             if (r < RPL) {
                 co.P[r][0] = P[0];
                 co.P[r][1] = P[1];
@@ -170,8 +172,10 @@ struct StreamLane {
             }
             uint32_t prev = dpp_from_prev_lane(A);
             uint32_t next = dpp_from_next_lane(A);
-            prev = xg == 0u ? 0u : prev;
-            next = xg == lpr - 1u ? 0u : next;
+            if constexpr (!FULLROW) {  // several x-rows share a 16-lane DPP row
+                prev = xg == 0u ? 0u : prev;
+                next = xg == lpr - 1u ? 0u : next;
+            }
             const uint32_t Lm = PLANAR ? shift_left_neighbor_nat(A, prev) : shift_left_neighbor_il(A, prev);
             const uint32_t Rm = PLANAR ? shift_right_neighbor_nat(A, next) : shift_right_neighbor_il(A, next);
             rs[r] = row_sums(A, Lm, Rm);
@@ -252,7 +256,7 @@ struct StreamLane {
 };
 
 
-template <uint32_t FLAGS, int RPL, bool PLANAR>
+template <uint32_t FLAGS, int RPL, bool PLANAR, bool FULLROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <= 2 ? 4 : 1))) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
                                                      StreamArgs a, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
@@ -273,7 +277,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
 
     if (gwave < a.total_waves) {
         const uint32_t buf_bytes = a.G * a.Y * a.Z * 16u;
-        StreamLane<FLAGS, RPL, PLANAR> L{__builtin_amdgcn_make_buffer_rsrc((void *)in, 0, (int)buf_bytes, kBufferDword3),
+        StreamLane<FLAGS, RPL, PLANAR, FULLROW> L{__builtin_amdgcn_make_buffer_rsrc((void *)in, 0, (int)buf_bytes, kBufferDword3),
                                          __builtin_amdgcn_make_buffer_rsrc((void *)out, 0, (int)buf_bytes, kBufferDword3),
                                          a};
         L.lane = threadIdx.x & 63u;
@@ -490,9 +494,14 @@ static void launch_flags(const uint4 *in, uint4 *out, const StreamArgs &a, const
 {
     const dim3 grid(p.blocks), block(64 * p.waves);
     switch (p.rpl) {
-    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
-    case 2: hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
-    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 2:
+        if (p.lpr == 16u)
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, true>), grid, block, 0, st, in, out, a, parts, done);
+        else
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done);
+        break;
+    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done); break;
     }
 }
 
