@@ -10,7 +10,9 @@
 // the host between ticks or between jobs.
 //
 // Cell storage per job (W = ceil(X/32) words per row, natural bit order):
-//   * 4 state bit planes, word w + W*(y + Y*z), updated in place;
+//   * 4 state bit planes, interleaved: the four plane words of cell word
+//     s = w + W*(y + Y*z) are words 4s .. 4s+3 (one 16-byte access), updated
+//     in place;
 //   * 2 live planes (ping-pong) with a zero guard band: rows y = -1 and Y and
 //     planes z = -1 and Z exist and stay 0, word w + W*((y+1) + (Y+2)*(z+1)),
 //     so neighbour reads never branch on the grid faces.
@@ -29,7 +31,7 @@ namespace cave {
 
 struct JobGeo {
     uint32_t X, Y, Z, W;
-    uint32_t pplane;     // W * Y * Z: one state plane
+    uint32_t pplane;     // W * Y * Z: cell words of one state plane
     uint32_t aplane;     // W * (Y+2) * (Z+2): one guarded live plane
     uint32_t aslab;      // W * (Y+2): words per z plane of a live plane
     uint32_t last_mask;  // valid cells of the last word of a row
@@ -107,18 +109,37 @@ __device__ __forceinline__ void st_live(T *p, uint32_t v, bool shared)
 typedef __attribute__((address_space(1))) uint32_t gword;
 typedef __attribute__((address_space(1))) uint8_t gbyte;
 
+// ld4 / st4: the four state planes of one cell word, 16 bytes at a word
+// offset that is a multiple of 4 (every storage base is 16-byte aligned).
 template <bool GLOBAL>
 struct Words {
     uint32_t *base;
     __device__ __forceinline__ uint32_t *at(uint32_t off) const { return base + off; }
+    __device__ __forceinline__ uint4 ld4(uint32_t off) const { return *reinterpret_cast<const uint4 *>(base + off); }
+    __device__ __forceinline__ void st4(uint32_t off, const uint4 &v) const
+    {
+        *reinterpret_cast<uint4 *>(base + off) = v;
+    }
 };
 
 template <>
 struct Words<true> {
+    typedef unsigned int quad __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) quad gquad;
     gword *base;
     __device__ __forceinline__ gword *at(uint32_t off) const
     {
         return reinterpret_cast<gword *>(reinterpret_cast<gbyte *>(base) + (off << 2));
+    }
+    __device__ __forceinline__ uint4 ld4(uint32_t off) const
+    {
+        const quad q = *reinterpret_cast<const gquad *>(reinterpret_cast<gbyte *>(base) + (off << 2));
+        return make_uint4(q.x, q.y, q.z, q.w);
+    }
+    __device__ __forceinline__ void st4(uint32_t off, const uint4 &v) const
+    {
+        const quad q = {v.x, v.y, v.z, v.w};
+        *reinterpret_cast<gquad *>(reinterpret_cast<gbyte *>(base) + (off << 2)) = q;
     }
 };
 
@@ -139,14 +160,14 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
     // zi = z0 - 1), advanced by one plane per step instead of recomputed with
     // multiplies:
     //   aoff: live word of row y-1 at plane zi:      acur + w + W*(y + (Y+2)*(zi+1))
-    //   poff: state word of row y at plane zi-1:     w + W*(y + Y*(zi-1))
+    //   poff: state words of row y at plane zi-1:    4 * (w + W*(y + Y*(zi-1)))
     //   noff: next live word of row y at plane zi-1: anext + w + W*((y+1) + (Y+2)*zi)
     // poff and noff wrap below plane 0 on the first steps; they are used only
     // once plane zi-1 is one of the item's planes.
     const uint32_t wy = (uint32_t)w + g.W * (uint32_t)y;
     const uint32_t pstride = g.W * g.Y;
     uint32_t aoff = acur + wy + g.aslab * (uint32_t)z0;
-    uint32_t poff = wy + pstride * (uint32_t)(z0 - 2);
+    uint32_t poff = 4u * (wy + pstride * (uint32_t)(z0 - 2));
     uint32_t noff = anext + wy + g.W + g.aslab * (uint32_t)(z0 - 1);
 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
@@ -157,18 +178,19 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
         if (shr) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const auto *p = S.at(aoff + (uint32_t)d * g.W);
-                av[d] = ld_live<COH>(p, true);
-                pv[d] = ld_live<COH>(p - 1, true);
-                nv[d] = ld_live<COH>(p + 1, true);
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * g.W);  // the word before the row's own
+                pv[d] = ld_live<COH>(q, true);
+                av[d] = ld_live<COH>(q + 1, true);
+                nv[d] = ld_live<COH>(q + 2, true);
             }
         } else {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const auto *p = S.at(aoff + (uint32_t)d * g.W);
-                av[d] = p[0];
-                pv[d] = p[-1];
-                nv[d] = p[1];
+                // one address per row: the word before the row's own, then +1, +2
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * g.W);
+                pv[d] = q[0];
+                av[d] = q[1];
+                nv[d] = q[2];
             }
         }
 #pragma unroll
@@ -189,9 +211,8 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
             const Bits<5> n = add2<5>(ci.Q, S1c);
             uint32_t tb, ts;
             apply_rule_rows<1>(&n, rc.born_lut, rc.surv_lut, &tb, &ts);
-            auto *P0 = S.at(poff), *P1 = S.at(poff + g.pplane), *P2 = S.at(poff + 2u * g.pplane),
-                 *P3 = S.at(poff + 3u * g.pplane);
-            uint32_t pl[4] = {*P0, *P1, *P2, *P3};
+            const uint4 pq = S.ld4(poff);
+            uint32_t pl[4] = {pq.x, pq.y, pq.z, pq.w};
             const uint32_t NZ = nonzero_mask(pl);
             const uint32_t Al = ci.A;
             const uint32_t set_m = vmask & ((Al & ts) | (~NZ & tb));
@@ -200,10 +221,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
             acc.born += __popc(set_m ^ surv);
             acc.dying += __popc(NZ & ~surv);
             update_planes(pl, set_m, NZ, rc.mk);
-            *P0 = pl[0];
-            *P1 = pl[1];
-            *P2 = pl[2];
-            *P3 = pl[3];
+            S.st4(poff, make_uint4(pl[0], pl[1], pl[2], pl[3]));
             st_live<COH>(S.at(noff), set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
             colmask |= set_m;
             if (set_m) {
@@ -212,7 +230,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
             }
         }
         aoff += g.aslab;
-        poff += pstride;
+        poff += 4u * pstride;
         noff += g.aslab;
     };
 
@@ -299,10 +317,8 @@ __device__ void init_cells(const DevJob &j, const JobGeo &g, const Words<GLOBAL>
             for (uint32_t x = xa; x < xb; x++)
                 if (init_alive(j.init, x, y, z)) bits |= 1u << (x - 32u * w);
         }
-        *S.at(idx) = (j.M & 1u) ? bits : 0u;
-        *S.at(g.pplane + idx) = (j.M & 2u) ? bits : 0u;
-        *S.at(2u * g.pplane + idx) = (j.M & 4u) ? bits : 0u;
-        *S.at(3u * g.pplane + idx) = (j.M & 8u) ? bits : 0u;
+        S.st4(4u * idx, make_uint4((j.M & 1u) ? bits : 0u, (j.M & 2u) ? bits : 0u, (j.M & 4u) ? bits : 0u,
+                                   (j.M & 8u) ? bits : 0u));
         st_live<COH>(S.at(acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u))), bits & live, true);
     }
 }
@@ -321,7 +337,7 @@ __device__ void store_final(const JobGeo &g, const uint32_t *S, uint32_t *out)
         uint32_t word = 0u;
 #pragma unroll
         for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t byte = (S[k * g.pplane + idx] >> sh) & 0xFFu;
+            const uint32_t byte = (S[4u * idx + k] >> sh) & 0xFFu;
 #pragma unroll
             for (uint32_t i = 0; i < 8; i++) word |= ((byte >> i) & 1u) << (4u * i + k);
         }
@@ -568,7 +584,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
 
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];  // 16-byte state accesses
     __shared__ BlockShared sh;
 
     uint32_t *arena = a.arena ? a.arena + (size_t)blockIdx.x * a.arena_words : nullptr;
@@ -636,7 +652,7 @@ __device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
 // closed (see serve_claim).
 __global__ void __launch_bounds__(1024) k_serve(BatchArgs a)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];  // 16-byte state accesses
     __shared__ BlockShared sh;
 
     constexpr uint32_t kJobWords = sizeof(DevJob) / 4u;

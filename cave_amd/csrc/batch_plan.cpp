@@ -371,7 +371,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             plan.region_words += (words[i] + 63u) / 64u * 64u;
             j.sync_slot = plan.nsync++;
         } else if (j.tier == kTierBlockArena) {
-            plan.arena_words = std::max(plan.arena_words, words[i]);
+            plan.arena_words = std::max(plan.arena_words, (words[i] + 63u) / 64u * 64u);  // 256-byte block strides
         }
         order[i] = {parts > 1 ? 0 : 1, t[i] / parts, i};
         longest_part = std::max(longest_part, t[i] / parts);

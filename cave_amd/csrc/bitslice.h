@@ -231,9 +231,15 @@ __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_
         u &= u - 1u;
         const uint32_t bm = 0u - ((born_lut >> v) & 1u);
         const uint32_t sm = 0u - ((surv_lut >> v) & 1u);
+        // inv[b] = ~0 where bit b of the count is 0.  The empty asm pins each
+        // mask to a scalar register: left to itself the compiler builds the
+        // top bit's mask as a vector select (one VALU op per count).
         uint32_t inv[5];
 #pragma unroll
-        for (int b = 0; b < 5; b++) inv[b] = ((v >> b) & 1u) ? 0u : ~0u;
+        for (int b = 0; b < 5; b++) {
+            inv[b] = ((v >> b) & 1u) - 1u;
+            asm volatile("" : "+s"(inv[b]));
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             uint32_t mt = n[r].b[0] ^ inv[0];
