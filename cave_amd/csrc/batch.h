@@ -101,12 +101,12 @@ struct ServeDev {
 };
 
 // Words of cell storage a job needs: 4 state planes of ceil(X/32) words per
-// row, 2 live planes with a one-row / one-plane zero guard band, and one
-// padding word past the end (read, masked, by the last x-neighbour load).
+// row, and 2 live planes with a one-row / one-plane zero guard band, a guard
+// word after each row and one before the plane.
 inline uint64_t job_words(uint32_t X, uint32_t Y, uint32_t Z)
 {
     const uint64_t W = (X + 31u) / 32u;
-    return 4ull * W * Y * Z + 2ull * W * (Y + 2ull) * (Z + 2ull) + 2ull;
+    return 4ull * W * Y * Z + 2ull * ((W + 1ull) * (Y + 2ull) * (Z + 2ull) + 1ull);
 }
 
 // Work split of one tick: columns (x-word, y) times z-chunks, about one item

@@ -14,8 +14,11 @@
 //     s = w + W*(y + Y*z) are words 4s .. 4s+3 (one 16-byte access), updated
 //     in place;
 //   * 2 live planes (ping-pong) with a zero guard band: rows y = -1 and Y and
-//     planes z = -1 and Z exist and stay 0, word w + W*((y+1) + (Y+2)*(z+1)),
-//     so neighbour reads never branch on the grid faces.
+//     planes z = -1 and Z exist and stay 0, each row ends with a zero guard
+//     word, and one zero word precedes the plane: word 1 + w + (W+1)*((y+1) +
+//     (Y+2)*(z+1)) of the plane.  The x-neighbour words of a row's first and
+//     last words are guard words, so neighbour reads never branch or mask on
+//     the grid faces.
 // A job whose storage fits the block's LDS keeps its whole grid in LDS for all
 // of its ticks ("tier 0"); a larger job runs in a per-block region of a global
 // arena ("tier 1", L2 / MALL resident at search grid sizes).  Both tiers run
@@ -32,8 +35,9 @@ namespace cave {
 struct JobGeo {
     uint32_t X, Y, Z, W;
     uint32_t pplane;     // W * Y * Z: cell words of one state plane
-    uint32_t aplane;     // W * (Y+2) * (Z+2): one guarded live plane
-    uint32_t aslab;      // W * (Y+2): words per z plane of a live plane
+    uint32_t aplane;     // (W+1) * (Y+2) * (Z+2) + 1: one guarded live plane
+    uint32_t aslab;      // (W+1) * (Y+2): words per z plane of a live plane
+    uint32_t alive0;     // first row word (w = 0, y = -1, z = -1) of live plane 0
     uint32_t last_mask;  // valid cells of the last word of a row
     uint32_t tzc;        // planes per z-chunk
     uint32_t zc0, nzc;   // this part's z-chunks [zc0, zc0 + nzc)
@@ -49,8 +53,9 @@ __device__ __forceinline__ JobGeo job_geo(const DevJob &j)
     g.Z = j.Z;
     g.W = (j.X + 31u) / 32u;
     g.pplane = g.W * j.Y * j.Z;
-    g.aslab = g.W * (j.Y + 2u);
-    g.aplane = g.aslab * (j.Z + 2u);
+    g.aslab = (g.W + 1u) * (j.Y + 2u);
+    g.aplane = g.aslab * (j.Z + 2u) + 1u;
+    g.alive0 = 4u * g.pplane + 1u;
     const uint32_t r = j.X - 32u * (g.W - 1u);
     g.last_mask = r >= 32u ? ~0u : ((1u << r) - 1u);
     const uint32_t zchunks = max(1u, j.zchunks);
@@ -143,32 +148,32 @@ struct Words<true> {
     }
 };
 
-// One item: column (w, y), planes [z0, z1).  A_cur / A_next / P are offsets
-// into S.  Loads of the x-neighbour words of a row always hit valid storage
-// (the previous / next row or a guard row); words outside the row are masked.
+// One item: column (w, y), planes [z0, z1).  acur / anext are the first row
+// words of the current / next live plane.  The x-neighbour words of a row
+// are the previous / next words, guard words (zero) at the row's ends.
 // COH: the job is split over several blocks (boundary planes are shared).
 template <uint32_t FLAGS, bool COH, bool GLOBAL>
 __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                                          const RuleConsts &rc, int w, int y, int z0, int z1, StatAcc &acc)
 {
     const uint32_t vmask = (uint32_t)w + 1u < g.W ? ~0u : g.last_mask;
-    const uint32_t pm = w > 0 ? ~0u : 0u;
-    const uint32_t nm = (uint32_t)w + 1u < g.W ? ~0u : 0u;
     uint32_t colmask = 0u;
     int zmin = 0x7fffffff, zmax = -1;
     // Running word offsets for the step at plane zi (the first step has
     // zi = z0 - 1), advanced by one plane per step instead of recomputed with
     // multiplies:
-    //   aoff: live word of row y-1 at plane zi:      acur + w + W*(y + (Y+2)*(zi+1))
+    //   aoff: live word of row y-1 at plane zi:      acur + w + (W+1)*(y + (Y+2)*(zi+1))
     //   poff: state words of row y at plane zi-1:    4 * (w + W*(y + Y*(zi-1)))
-    //   noff: next live word of row y at plane zi-1: anext + w + W*((y+1) + (Y+2)*zi)
+    //   noff: next live word of row y at plane zi-1: anext + w + (W+1)*((y+1) + (Y+2)*zi)
     // poff and noff wrap below plane 0 on the first steps; they are used only
     // once plane zi-1 is one of the item's planes.
     const uint32_t wy = (uint32_t)w + g.W * (uint32_t)y;
     const uint32_t pstride = g.W * g.Y;
-    uint32_t aoff = acur + wy + g.aslab * (uint32_t)z0;
+    const uint32_t arow = g.W + 1u;  // live row stride
+    const uint32_t wya = (uint32_t)w + arow * (uint32_t)y;
+    uint32_t aoff = acur + wya + g.aslab * (uint32_t)z0;
     uint32_t poff = 4u * (wy + pstride * (uint32_t)(z0 - 2));
-    uint32_t noff = anext + wy + g.W + g.aslab * (uint32_t)(z0 - 1);
+    uint32_t noff = anext + wya + arow + g.aslab * (uint32_t)(z0 - 1);
 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
         RowSums rs[3];
@@ -178,7 +183,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
         if (shr) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const auto *q = S.at(aoff - 1u + (uint32_t)d * g.W);  // the word before the row's own
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * arow);  // the word before the row's own
                 pv[d] = ld_live<COH>(q, true);
                 av[d] = ld_live<COH>(q + 1, true);
                 nv[d] = ld_live<COH>(q + 2, true);
@@ -187,7 +192,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 // one address per row: the word before the row's own, then +1, +2
-                const auto *q = S.at(aoff - 1u + (uint32_t)d * g.W);
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * arow);
                 pv[d] = q[0];
                 av[d] = q[1];
                 nv[d] = q[2];
@@ -195,8 +200,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
         }
 #pragma unroll
         for (int d = 0; d < 3; d++)
-            rs[d] = row_sums(av[d], shift_left_neighbor_nat(av[d], pv[d] & pm),
-                             shift_right_neighbor_nat(av[d], nv[d] & nm));
+            rs[d] = row_sums(av[d], shift_left_neighbor_nat(av[d], pv[d]), shift_right_neighbor_nat(av[d], nv[d]));
         const Bits<4> S1c = add3<4>(kind2<FLAGS>(rs[0]), kind1<FLAGS>(rs[1]), kind2<FLAGS>(rs[2]));
         if constexpr (kBoxSum<FLAGS>) {
             co.Q = add2<5>(ci.S1p, S1c);
@@ -297,9 +301,11 @@ __device__ void init_cells(const DevJob &j, const JobGeo &g, const Words<GLOBAL>
     const uint32_t s1 = j.part + 1u >= parts ? g.Z + 2u : g.zhi + 1u;
     const uint32_t nclr = (s1 > s0 ? s1 - s0 : 0u) * g.aslab;
     for (uint32_t i = threadIdx.x; i < nclr; i += blockDim.x) {
-        st_live<COH>(S.at(4u * g.pplane + s0 * g.aslab + i), 0u, true);
-        st_live<COH>(S.at(4u * g.pplane + g.aplane + s0 * g.aslab + i), 0u, true);
+        st_live<COH>(S.at(g.alive0 + s0 * g.aslab + i), 0u, true);
+        st_live<COH>(S.at(g.alive0 + g.aplane + s0 * g.aslab + i), 0u, true);
     }
+    if (j.part == 0u && threadIdx.x < 2u)  // the word before each live plane
+        st_live<COH>(S.at(g.alive0 - 1u + threadIdx.x * g.aplane), 0u, true);
     __syncthreads();
     const uint32_t live = j.M <= 15u ? ~0u : 0u;
     const uint32_t base = g.W * g.Y * g.zlo;
@@ -319,7 +325,7 @@ __device__ void init_cells(const DevJob &j, const JobGeo &g, const Words<GLOBAL>
         }
         S.st4(4u * idx, make_uint4((j.M & 1u) ? bits : 0u, (j.M & 2u) ? bits : 0u, (j.M & 4u) ? bits : 0u,
                                    (j.M & 8u) ? bits : 0u));
-        st_live<COH>(S.at(acur + w + g.W * ((y + 1u) + (g.Y + 2u) * (z + 1u))), bits & live, true);
+        st_live<COH>(S.at(acur + w + (g.W + 1u) * ((y + 1u) + (g.Y + 2u) * (z + 1u))), bits & live, true);
     }
 }
 
@@ -482,7 +488,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     const RuleConsts rc = rule_consts_for(j.flags, j.M, j.born_lut, j.surv_lut);
     const bool multi = j.parts > 1u;
     uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
-    uint32_t acur = 4u * g.pplane, anext = 4u * g.pplane + g.aplane;
+    uint32_t acur = g.alive0, anext = g.alive0 + g.aplane;
     const Words<false> SL{lds};
     const Words<true> SG{(gword *)S};  // generic -> global address space
     if (j.tier == kTierLds) init_cells<false>(j, g, SL, acur);
