@@ -365,18 +365,27 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max
 
 def copy_calibration(mib: int, copies: int):
     """K plain device-to-device copies of a MIB-MiB buffer: each reads and
-    writes exactly MIB MiB, the yardstick for the HBM byte counters."""
+    writes exactly MIB MiB, the yardstick for the HBM byte counters.  Also
+    times them (HIP events on torch's stream, after 5 untimed copies): the
+    best a kernel moving the same bytes can expect."""
     import torch
 
     n = mib << 20
     src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    for _ in range(copies):
+    for _ in range(5):
         dst.copy_(src)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(copies):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / max(1, copies)
     print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n, "bytes_written_per_copy": n,
-                                           "copies": copies}}), flush=True)
+                                           "copies": copies, "us_per_copy": us,
+                                           "tb_per_s_read_plus_write": 2 * n / us / 1e6}}), flush=True)
 
 
 def main():
