@@ -48,6 +48,10 @@ struct JobSync {
     uint32_t sum[3][4];     // dying, born, surviving
     int32_t nmin[3][4];     // 0x7fffffff - bbox min (max-combined, 0 = empty)
     int32_t xmax1[3][4];    // bbox max + 1 (max-combined, 0 = empty)
+    // per-tick rendezvous of jobs whose totals fit a word: surviving (bits
+    // 0-23) | born (24-47) | arrivals (48-63), one slot per tick mod 3
+    uint64_t tick[3];
+    uint64_t pad2_;
 };
 
 // Result record written by the kernel (host converts to CaveSimResult).

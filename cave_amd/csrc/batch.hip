@@ -464,6 +464,49 @@ __device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t ph
     return *s_flag != 0u;
 }
 
+// The per-tick rendezvous of a split job whose totals fit one 64-bit word
+// (fewer than 2^24 cells, no per-tick rows, the reference's death rule): each
+// part adds surviving | born << 24 | 1 << 48 to the tick's slot with one
+// device-scope atomic, then polls the slot until every part has arrived; the
+// word then holds the job's surviving and born totals, which are all the
+// manager needs per tick.  (The general rendezvous above waits for its stat
+// atomics, then arrives, polls and reads the totals back: two more round
+// trips per tick.)  The slot of phase + 1 (= phase - 2) was last read before
+// every part arrived for phase - 1: part 0 clears it before it arrives.
+__device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint32_t phase, uint32_t born,
+                                     uint32_t surv, uint32_t *born_tot, uint32_t *surv_tot, uint32_t *s_flag)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        JobSync *js = a.sync + j.sync_slot;
+        if (j.part == 0u && phase >= 2u) {
+            __hip_atomic_store(&js->tick[(phase + 1u) % 3u], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        uint64_t *w = &js->tick[phase % 3u];
+        const uint64_t add = (uint64_t)surv | ((uint64_t)born << 24) | ((uint64_t)1 << 48);
+        uint64_t v = __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
+        const uint64_t t0 = wall_clock64();
+        uint32_t ok = 1u;
+        while ((uint32_t)(v >> 48) < j.parts) {
+            if (__hip_atomic_load(&js->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > a.wait_limit) {
+                __hip_atomic_store(&js->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0u;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        *surv_tot = (uint32_t)(v & 0xFFFFFFu);
+        *born_tot = (uint32_t)((v >> 24) & 0xFFFFFFu);
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0u;
+}
+
 // Block-wide scratch of the batched kernels.
 struct BlockShared {
     uint32_t sum[3 * 16];
@@ -502,9 +545,16 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     bool ok = true;
     if (multi) ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
+    // Split jobs whose per-tick totals fit one word take the one-atomic
+    // per-tick rendezvous; their dying totals and the last tick's bounding box
+    // are combined once, after the last tick.
+    const bool packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
+                        j.parts < (1u << 16);
+    uint32_t part_dying_all = 0u, part_dying_last = 0u;  // thread 0, packed jobs
     uint32_t ticks_run = 0;  // the same in every thread and every part
+    StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
-        StatAcc acc;
+        acc = StatAcc{};
         if (j.tier == kTierLds) run_tick<false>(j.flags, g, SL, acur, anext, rc, acc);
         else if (multi) run_tick<true>(j.flags, g, SG, acur, anext, rc, acc);
         else run_tick<false>(j.flags, g, SG, acur, anext, rc, acc);
@@ -512,14 +562,24 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         uint32_t tot[3];
         int32_t bmin[3], bmax[3];
         block_sum3(acc.dying, acc.born, acc.surv, sh.sum, tot);
-        if (multi) {
+        if (packed) {
+            if (threadIdx.x == 0) {
+                part_dying_all += tot[0];
+                part_dying_last = tot[0];
+            }
+            ok = part_rendezvous_tick(a, j, t + 1u, tot[1], tot[2], &tot[1], &tot[2], &sh.ok);
+            if (!ok) break;
+        } else if (multi) {
             // every tick: whether this tick is the last is known only after the totals
             block_bbox(acc, sh.box, bmin, bmax);
             ok = part_rendezvous(a, j, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
             if (!ok) break;
         }
         if (threadIdx.x == 0) {
-            const uint32_t dying = tot[0], born = tot[1], surv = tot[2];
+            // packed jobs learn the dying total after the last tick: it is
+            // taken out of the died total and the last row then (the u32
+            // totals wrap like the reference's, so the order does not matter)
+            const uint32_t dying = packed ? 0u : tot[0], born = tot[1], surv = tot[2];
             const uint32_t dead = cells - dying - born - surv;
             // searchManager.slang:115-139
             info.numberOfSimulationTicksSurvived++;
@@ -543,7 +603,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         __syncthreads();
         // the bounding box is reduced only when someone reads it
         if (!multi && (sh.done || a.per_tick)) block_bbox(acc, sh.box, bmin, bmax);
-        if (threadIdx.x == 0 && (sh.done || a.per_tick)) {
+        if (threadIdx.x == 0 && !packed && (sh.done || a.per_tick)) {
             for (int k = 0; k < 3; k++) {
                 last.bbox_min[k] = bmin[k];
                 last.bbox_max[k] = bmax[k];
@@ -557,6 +617,26 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         __syncthreads();
         if (done) break;
     }
+    // the next phase of the general rendezvous
+    uint32_t phase = packed ? 1u : ticks_run + 1u;
+    if (packed && ok && ticks_run > 0u) {
+        int32_t bmin[3], bmax[3];
+        block_bbox(acc, sh.box, bmin, bmax);
+        uint32_t fin[3] = {part_dying_all, part_dying_last, 0u};
+        ok = part_rendezvous(a, j, phase, fin, bmin, bmax, fin, bmin, bmax, &sh.ok);
+        phase++;
+        if (ok && threadIdx.x == 0) {
+            const uint32_t born = last.snap.numberOfCellsBorn, surv = last.snap.numberOfSurvivingCells;
+            info.totalNumberOfCellsThatDied -= fin[0];
+            last.snap.numberOfDyingCells = fin[1];
+            last.snap.numberOfDeadCells = cells - fin[1] - born - surv;
+            last.changed = born + fin[1];
+            for (int k = 0; k < 3; k++) {
+                last.bbox_min[k] = bmin[k];
+                last.bbox_max[k] = bmax[k];
+            }
+        }
+    }
     if (j.has_final && ok) {
         uint32_t *dst = reinterpret_cast<uint32_t *>(j.final_addr);
         if (j.tier == kTierLds) store_final(g, lds, dst);
@@ -565,7 +645,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     if constexpr (SERVE) {
         // the final cells of every part reach memory before part 0 reports
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        if (multi && ok) ok = part_rendezvous(a, j, ticks_run + 1u, t3, b3, c3, t3, b3, c3, &sh.ok);
+        if (multi && ok) ok = part_rendezvous(a, j, phase, t3, b3, c3, t3, b3, c3, &sh.ok);
         else __syncthreads();
     }
     if (threadIdx.x == 0 && j.part == 0u) {

@@ -142,6 +142,32 @@ def test_split_simulations_match_whole():
     check(sims[1::2], cv.ENGINE_PERSISTENT)
 
 
+def test_split_simulations_without_rows():
+    # without per-tick rows, split jobs take the one-word per-tick rendezvous
+    # (surviving / born totals only; dying totals and the bounding box are
+    # combined after the last tick): every result field, the last row and the
+    # final cells equal one block per grid and the oracle, with and without
+    # early death
+    dims = [(70, 40, 30), (100, 33, 21), (65, 64, 64), (96, 50, 37), (128, 41, 77)]
+    sims = random_sims(10, 6, dims, ticks=14)
+    for s in sims[::2]:
+        s.sim_flags |= cv.SIM_NO_EARLY_EXIT
+
+    def key(r):
+        return (r.info.totalNumberOfCellsThatDied, r.info.totalNumberOfCellsThatWereBorn,
+                r.info.numberOfSimulationTicksSurvived, r.info.status, r.info.maxAliveCount, r.info.finalAliveCount,
+                r.classification, r.ticks_run, r.last.as_tuple(), r.final_hash)
+
+    print("whole", flush=True)
+    whole, _, _ = cv.batch_run(sims, max_parts=1)
+    for parts in (2, 3, 7, 16):
+        print(f"max_parts={parts}", flush=True)
+        split, _, _ = cv.batch_run(sims, max_parts=parts)
+        for i in range(len(sims)):
+            assert key(split[i]) == key(whole[i]), f"sim {i} parts {parts}"
+    check(sims, cv.ENGINE_PERSISTENT, per_tick=False)
+
+
 def test_autotuned_uneven_batch():
     # the autotuner picks block size, items per thread and splits; results
     # equal the fixed single-block layout and the oracle
