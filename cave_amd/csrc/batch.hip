@@ -545,11 +545,16 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     bool ok = true;
     if (multi) ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
-    // Split jobs whose per-tick totals fit one word take the one-atomic
-    // per-tick rendezvous; their dying totals and the last tick's bounding box
-    // are combined once, after the last tick.
-    const bool packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
-                        j.parts < (1u << 16);
+    // Served split requests whose per-tick totals fit one word take the
+    // one-atomic per-tick rendezvous; their dying totals and the last tick's
+    // bounding box are combined once, after the last tick.  The batch kernel
+    // keeps the general rendezvous: compiled into it, this path costs the
+    // one-block-per-job code ~6% (49^3 search), more than the split jobs of an
+    // uneven batch gain (~5%).
+    bool packed = false;
+    if constexpr (SERVE)
+        packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
+                 j.parts < (1u << 16);
     uint32_t part_dying_all = 0u, part_dying_last = 0u;  // thread 0, packed jobs
     uint32_t ticks_run = 0;  // the same in every thread and every part
     StatAcc acc;
