@@ -351,7 +351,8 @@ __device__ void store_final(const JobGeo &g, const uint32_t *S, uint32_t *out)
     }
 }
 
-// Block sums of three counters; thread 0 receives them.  scratch: 3 words per wave.
+// Block sums of three counters; wave 0 (thread 0 among it) receives them.
+// scratch: 3 words per wave.
 __device__ __forceinline__ void block_sum3(uint32_t a, uint32_t b, uint32_t c, uint32_t *scratch, uint32_t out[3])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -364,17 +365,17 @@ __device__ __forceinline__ void block_sum3(uint32_t a, uint32_t b, uint32_t c, u
         scratch[3 * wave + 2] = c;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        out[0] = out[1] = out[2] = 0u;
-        for (int i = 0; i < nw; i++) {
-            out[0] += scratch[3 * i];
-            out[1] += scratch[3 * i + 1];
-            out[2] += scratch[3 * i + 2];
-        }
+    if (wave == 0) {  // lane i reduces wave i's partials (nw <= 16)
+        const bool has = lane < nw;
+        const uint32_t *v = scratch + 3 * (has ? lane : 0);
+        out[0] = wave_sum_u32(has ? v[0] : 0u);
+        out[1] = wave_sum_u32(has ? v[1] : 0u);
+        out[2] = wave_sum_u32(has ? v[2] : 0u);
     }
 }
 
-// Block min / max of the bounding box; thread 0 receives it.  scratch: 6 words per wave.
+// Block min / max of the bounding box; wave 0 (thread 0 among it) receives
+// it.  scratch: 6 words per wave.
 __device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch, int32_t bmin[3], int32_t bmax[3])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -386,16 +387,13 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch,
     if (lane == 0)
         for (int k = 0; k < 6; k++) scratch[6 * wave + k] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wave == 0) {  // lane i reduces wave i's partials
+        const bool has = lane < nw;
+        const int32_t *p = scratch + 6 * (has ? lane : 0);
         for (int k = 0; k < 3; k++) {
-            bmin[k] = 0x7fffffff;
-            bmax[k] = -1;
+            bmin[k] = wave_min_i32(has ? p[k] : 0x7fffffff);
+            bmax[k] = wave_max_i32(has ? p[3 + k] : -1);
         }
-        for (int i = 0; i < nw; i++)
-            for (int k = 0; k < 3; k++) {
-                bmin[k] = min(bmin[k], scratch[6 * i + k]);
-                bmax[k] = max(bmax[k], scratch[6 * i + 3 + k]);
-            }
     }
 }
 

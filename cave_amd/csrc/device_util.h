@@ -47,26 +47,36 @@ __host__ __device__ inline bool init_alive(const DevInit &in, uint32_t x, uint32
     return cell_hash(x, y, z, in.seed) < in.threshold;
 }
 
-// ---- wavefront reductions (64 lanes) via DPP-backed cross-lane ops ----
+// ---- wavefront reductions (64 full lanes) with DPP ----
+// No LDS traffic: two quad swaps and two 16-lane row rotations reduce each
+// row, then row broadcasts of lanes 15 and 31 fold the rows into lane 63,
+// whose value is returned to every lane.  Lanes a row mask leaves out take
+// `ident`, the operation's identity.
+template <typename Op>
+__device__ __forceinline__ int32_t wave_reduce_dpp(int32_t v, int32_t ident, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ inline uint32_t wave_sum_u32(uint32_t v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return (uint32_t)wave_reduce_dpp((int32_t)v, 0, [](int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); });
 }
 
 __device__ inline int32_t wave_min_i32(int32_t v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
+    return wave_reduce_dpp(v, 0x7fffffff, [](int32_t a, int32_t b) { return min(a, b); });
 }
 
 __device__ inline int32_t wave_max_i32(int32_t v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
+    return wave_reduce_dpp(v, (int32_t)0x80000000, [](int32_t a, int32_t b) { return max(a, b); });
 }
 
 // Thread-local statistics accumulator.
@@ -93,21 +103,19 @@ __device__ inline void block_reduce_stats(const StatAcc &a, StatPart *out, int32
         w[4] = m0; w[5] = m1; w[6] = m2; w[7] = x0; w[8] = x1; w[9] = x2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wave == 0) {  // lane w reduces wave w's partials (nwaves <= 64)
+        const bool has = lane < nwaves;
+        const int32_t *v = scratch + (has ? lane : 0) * 10;
         StatPart p;
-        p.dead = p.dying = p.born = p.surv = 0;
-        p.bmin[0] = p.bmin[1] = p.bmin[2] = 0x7fffffff;
-        p.bmax[0] = p.bmax[1] = p.bmax[2] = -1;
-        for (int w = 0; w < nwaves; w++) {
-            const int32_t *v = scratch + w * 10;
-            p.dead += (uint32_t)v[0]; p.dying += (uint32_t)v[1];
-            p.born += (uint32_t)v[2]; p.surv += (uint32_t)v[3];
-            for (int k = 0; k < 3; k++) {
-                p.bmin[k] = min(p.bmin[k], v[4 + k]);
-                p.bmax[k] = max(p.bmax[k], v[7 + k]);
-            }
+        p.dead = wave_sum_u32(has ? (uint32_t)v[0] : 0u);
+        p.dying = wave_sum_u32(has ? (uint32_t)v[1] : 0u);
+        p.born = wave_sum_u32(has ? (uint32_t)v[2] : 0u);
+        p.surv = wave_sum_u32(has ? (uint32_t)v[3] : 0u);
+        for (int k = 0; k < 3; k++) {
+            p.bmin[k] = wave_min_i32(has ? v[4 + k] : 0x7fffffff);
+            p.bmax[k] = wave_max_i32(has ? v[7 + k] : -1);
         }
-        *out = p;
+        if (lane == 0) *out = p;
     }
 }
 
