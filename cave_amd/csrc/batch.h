@@ -135,6 +135,9 @@ inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t thread
 const void *batch_kernel();
 const void *serve_kernel();
 int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
-int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
+// split_jobs: the batch holds jobs split over several blocks (selects the
+// kernel form with the one-word per-tick rendezvous).
+int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
+                 bool split_jobs);
 
 }  // namespace cave

@@ -521,7 +521,8 @@ struct BlockShared {
 // budget), the result record (part 0) and the final cells.  SERVE: the result
 // and completion flag go to host-coherent memory, after every store of the
 // simulation is visible to the host.
-template <bool SERVE>
+// FAST_SPLIT: split jobs may take the one-word per-tick rendezvous.
+template <bool SERVE, bool FAST_SPLIT>
 __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
 {
     const uint64_t t_start = wall_clock64();
@@ -543,14 +544,14 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     bool ok = true;
     if (multi) ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
-    // Served split requests whose per-tick totals fit one word take the
-    // one-atomic per-tick rendezvous; their dying totals and the last tick's
-    // bounding box are combined once, after the last tick.  The batch kernel
-    // keeps the general rendezvous: compiled into it, this path costs the
-    // one-block-per-job code ~6% (49^3 search), more than the split jobs of an
-    // uneven batch gain (~5%).
+    // Split jobs whose per-tick totals fit one word take the one-atomic
+    // per-tick rendezvous; their dying totals and the last tick's bounding box
+    // are combined once, after the last tick.  Only kernels built with
+    // FAST_SPLIT have this path: compiled in, it costs the one-block-per-job
+    // code ~6% (49^3 search), so batches without split jobs run a kernel
+    // without it.
     bool packed = false;
-    if constexpr (SERVE)
+    if constexpr (FAST_SPLIT)
         packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
                  j.parts < (1u << 16);
     uint32_t part_dying_all = 0u, part_dying_last = 0u;  // thread 0, packed jobs
@@ -671,6 +672,8 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     __syncthreads();
 }
 
+// SPLIT: the batch holds split jobs (FAST_SPLIT entry code).
+template <bool SPLIT>
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];  // 16-byte state accesses
@@ -684,7 +687,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];
-        run_entry<false>(a, j, lds, arena, sh);
+        run_entry<false, SPLIT>(a, j, lds, arena, sh);
     }
 }
 
@@ -766,7 +769,7 @@ __global__ void __launch_bounds__(1024) k_serve(BatchArgs a)
             reinterpret_cast<uint32_t *>(&j)[k] =
                 __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t *>(&sh.job)[k]);
         __syncthreads();
-        run_entry<true>(a, j, lds, nullptr, sh);
+        run_entry<true, true>(a, j, lds, nullptr, sh);
     }
 }
 
@@ -781,13 +784,15 @@ int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t
     return CAVE_OK;
 }
 
-const void *batch_kernel() { return (const void *)k_batch; }
+const void *batch_kernel() { return (const void *)k_batch<false>; }
 
-int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st)
+int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
+                 bool split_jobs)
 {
-    if (lds_bytes > 64u * 1024u)
-        CAVE_HIP(hipFuncSetAttribute((const void *)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(threads), lds_bytes, st, a);
+    const void *k = split_jobs ? (const void *)k_batch<true> : (const void *)k_batch<false>;
+    if (lds_bytes > 64u * 1024u) CAVE_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    if (split_jobs) hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(threads), lds_bytes, st, a);
+    else hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(threads), lds_bytes, st, a);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

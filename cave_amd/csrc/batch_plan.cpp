@@ -438,7 +438,7 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
     CAVE_HIP(hipEventCreate(&e0));
     CAVE_HIP(hipEventCreate(&e1));
     CAVE_HIP(hipEventRecord(e0, st));
-    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st);
+    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st, plan.nsync > 0);
     if (rc) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
