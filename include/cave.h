@@ -47,7 +47,7 @@ extern "C" {
 enum {
     CAVE_OK = 0,
     CAVE_E_INVALID = 1,     /* bad argument / spec                      */
-    CAVE_E_UNSUPPORTED = 2, /* valid in the reference, not built here   */
+    CAVE_E_UNSUPPORTED = 2, /* valid in the reference, not in the form asked for here */
     CAVE_E_NOMEM = 3,       /* device or host allocation failed         */
     CAVE_E_DEVICE = 4,      /* HIP runtime error                        */
     CAVE_E_BUDGET = 5,      /* request larger than the admission budget */
