@@ -177,8 +177,10 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
 
     auto step = [&](int zi, const BCarry &ci, BCarry &co) {
         RowSums rs[3];
-        // planes zlo-1, zlo, zhi-1, zhi are written or read by a neighbouring part
-        const bool shr = COH && (zi <= (int)g.zlo || zi + 1 >= (int)g.zhi);
+        // planes zlo-1 and zhi are written by the neighbouring parts: read
+        // coherently.  This part's own planes, zlo and zhi-1 included, were
+        // written by this block, whose writes it sees after a barrier.
+        const bool shr = COH && (zi == (int)g.zlo - 1 || zi == (int)g.zhi);
         uint32_t av[3], pv[3], nv[3];
         if (shr) {
 #pragma unroll
