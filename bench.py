@@ -13,7 +13,8 @@ The same JSON line also carries the rule-search rate at 49^3 x 20 ticks
 (2048 configs, edge 24..160) with its per-bucket achieved fractions, online
 serving (Poisson 2000 requests/s for 60 s into the resident kernel: throughput,
 p50 / p99 latency, oracle spot checks) and the host-CPU baselines, each timed
-on a bounded sample with the CPU oracle.
+on a bounded sample with the fast CPU form (bit-sliced, threaded), which is
+itself checked against the scalar oracle and the GPU results.
 """
 from __future__ import annotations
 
@@ -88,26 +89,77 @@ def run_step_bench(steps: int, warmup: int):
     return wall, dev_ms, kernel
 
 
-def cpu_step_baseline(target_s: float = 12.0):
-    """The CPU oracle (scalar, 1 thread) on a bounded sample of the same
-    workload: the 512^3 grid, as many whole ticks as fit ~target_s."""
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def oracle_module():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    return O
+
+
+def fast_sims(specs):
+    """The fast CPU form's simulation list (oracle SIM_NP records) for specs."""
+    import numpy as np
+
+    O = oracle_module()
+    a = np.zeros(len(specs), dtype=O.SIM_NP)
+    for k, f in enumerate(("X", "Y", "Z")):
+        a[f] = specs["dims"][:, k]
+    for k, f in enumerate(("sx", "sy", "sz")):
+        a[f] = specs["spawn"][:, k]
+    a["max_ticks"] = specs["ticks"]
+    for f in ("M", "flags", "birth", "survival", "spawn_mode", "seed", "density"):
+        a[f] = specs[f]
+    return a
+
+
+def cpu_step_baseline(target_s: float = 10.0):
+    """Host-CPU baseline on the same workload: the fast CPU form (bit-sliced
+    64-cell words, z slabs over threads) steps the 512^3 grid for about
+    target_s.  It is checked against the scalar oracle on the first tick of
+    a 256^3 grid with the same rule and initial density; that tick also times
+    the scalar oracle (one thread)."""
+    import numpy as np
+
+    O = oracle_module()
     birth, surv = workload_rule()
-    cells = O.init_cells(GRID, GRID, GRID, 7, (GRID, GRID, GRID), 0, 0.2, 42)
-    w0 = (6 << 60) | birth
-    w1 = (7 << 60) | surv
-    ticks = 0
+    n = cpu_threads()
+    E = GRID // 2
+    small = O.init_cells(E, E, E, 7, (E, E, E), 0, 0.2, 42)
     t0 = time.perf_counter()
+    ref, _ = O.step(small, E, E, E, (6 << 60) | birth, (7 << 60) | surv)
+    scalar_s = time.perf_counter() - t0
+    g = O.FastGrid(small, E, E, E)
+    g.run(7, 7, birth, surv, 1, n_threads=n, stats=False)
+    assert np.array_equal(g.cells(), ref), "fast CPU form / scalar oracle disagree"
+    g.close()
+    cells = O.init_cells(GRID, GRID, GRID, 7, (GRID, GRID, GRID), 0, 0.2, 42)
+    g = O.FastGrid(cells, GRID, GRID, GRID)
+    del cells
+    ticks, t0 = 0, time.perf_counter()
     while True:
-        cells, _ = O.step(cells, GRID, GRID, GRID, w0, w1)
-        ticks += 1
-        if time.perf_counter() - t0 >= target_s or ticks >= 4:
+        g.run(7, 7, birth, surv, 4, n_threads=n, stats=False)
+        ticks += 4
+        if time.perf_counter() - t0 >= target_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": GRID ** 3 * ticks / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{ticks} tick(s) of the 512^3 FEV C=8 workload, scalar oracle, 1 thread"}
+    g.close()
+    return {"value": GRID ** 3 * ticks / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
+            "sample": f"{ticks} ticks of the 512^3 FEV C=8 workload, fast CPU form (bit-sliced 64-cell words, "
+                      f"z slabs over {n} threads); checked equal to the scalar oracle on the first tick of a "
+                      f"{E}^3 grid of the same rule and density",
+            "scalar_oracle": {"value": E ** 3 / scalar_s, "unit": "cell-updates/s", "cores": 1,
+                              "sample": f"1 tick of a {E}^3 grid of the same rule and density, scalar per-cell "
+                                        "oracle"}}
 
 
 def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
@@ -148,25 +200,16 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
                               "launch per sim per tick"}
     out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-
-        n_cpu, t0, cpu_updates, done = 0, time.perf_counter(), 0, 0
-        inits = {}
-        while time.perf_counter() - t0 < 10.0 and done < len(specs):
-            sp = specs[done]
-            M = int(sp["M"])
-            key = (M, int(sp["seed"]))
-            if key not in inits:
-                inits[key] = O.init_cells(49, 49, 49, M, (12, 12, 12), 0, 0.3, int(sp["seed"]))
-            t, info, _, _ = O.run_rule(49, 49, 49, M, 3, int(sp["birth"]), int(sp["survival"]), 20, inits[key])
-            cpu_updates += t * cells
-            done += 1
-        dt = time.perf_counter() - t0
+        O = oracle_module()
+        n = cpu_threads()
+        ticks, _, done, cpu_updates, dt = O.run_sims_fast(fast_sims(specs), n_threads=n, budget_s=10.0)
+        ran = ticks != 0xFFFFFFFF
+        assert np.array_equal(ticks[ran], res["ticks_run"][ran]), "fast CPU form / GPU disagree"
         out["cpu_baseline"] = {"value": done / seeds / dt, "unit": "configs/s",
-                               "cell_updates_per_s": cpu_updates / dt, "cores": 1, "kind": "port",
-                               "sample": f"first {done} sims ({done / seeds:.2f} configs) of the same list, "
-                                         "scalar oracle, 1 thread"}
+                               "cell_updates_per_s": cpu_updates / dt, "cores": n, "kind": "port",
+                               "sample": f"{done} sims ({done / seeds:.1f} configs) of the same list in list order, "
+                                         f"fast CPU form, one sim per thread on {n} threads; ticks run equal "
+                                         "the GPU's"}
     return out
 
 
@@ -227,28 +270,16 @@ def run_uneven_bench(n_configs=2048, cpu=True):
     }
     log(f"  timed call {dev_ms:.2f} ms")
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-
-        order = np.argsort(ups, kind="stable")  # cheapest first: many whole simulations in the sample
-        t0, done, cpu_ups = time.perf_counter(), 0, 0
-        for i in order:
-            if time.perf_counter() - t0 > 10.0:
-                break
-            sp = specs[i]
-            X, Y, Z = (int(v) for v in sp["dims"])
-            init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
-                                float(sp["density"]), int(sp["seed"]))
-            t, info, _, _ = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
-                                       int(sp["survival"]), int(sp["ticks"]), init)
-            assert t == res["ticks_run"][i], "CPU oracle / GPU disagree"
-            cpu_ups += t * X * Y * Z
-            done += 1
-        dt = time.perf_counter() - t0
+        O = oracle_module()
+        n = cpu_threads()
+        ticks, _, done, cpu_ups, dt = O.run_sims_fast(fast_sims(specs), n_threads=n, budget_s=10.0)
+        ran = ticks != 0xFFFFFFFF
+        assert np.array_equal(ticks[ran], res["ticks_run"][ran]), "fast CPU form / GPU disagree"
         cpu_rate = cpu_ups / dt
         out["cpu_baseline"] = {"value": cpu_rate / (ups.sum() / n_configs), "unit": "configs/s",
-                               "cell_updates_per_s": cpu_rate, "cores": 1, "kind": "port",
-                               "sample": f"{done} cheapest simulations of the list, scalar oracle, 1 thread; "
+                               "cell_updates_per_s": cpu_rate, "cores": n, "kind": "port",
+                               "sample": f"{done} simulations of the list in list order, fast CPU form, one "
+                                         f"simulation per thread on {n} threads; ticks run equal the GPU's; "
                                          "configs/s = CPU cell-updates/s / mean executed cell-updates per config"}
     return out
 
@@ -313,8 +344,7 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max
         "peak_resident_bytes": st["peak_resident_bytes"], "hbm_budget_bytes": 200 << 30,
     }
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
+        O = oracle_module()
 
         def check(i):
             sp = specs[i]
@@ -451,7 +481,7 @@ def main():
     }
     model, ncpu = cpu_info()
     if not a.no_cpu:
-        log("CPU oracle on the single-grid workload")
+        log("host-CPU baseline on the single-grid workload")
         cb = cpu_step_baseline()
         cb["host"] = f"{model} ({ncpu} logical CPUs visible)"
         out["cpu_baseline"] = cb

@@ -34,6 +34,14 @@ class Permutation(C.Structure):
                 ("survivalAndNeighborhoodRules", C.c_uint64)]
 
 
+class Sim(C.Structure):
+    """ofs_sim (cpu_fast.h): one simulation of a list for the fast CPU form."""
+    _fields_ = [(n, C.c_uint32) for n in ("X", "Y", "Z", "M", "flags", "max_ticks")] + \
+               [("birth", C.c_uint64), ("survival", C.c_uint64)] + \
+               [(n, C.c_uint32) for n in ("sx", "sy", "sz", "spawn_mode", "seed", "reserved")] + \
+               [("density", C.c_double)]
+
+
 class TickStats(C.Structure):
     _fields_ = [("snap", Snapshot), ("live", C.c_uint32), ("changed", C.c_uint32),
                 ("bbox_min", C.c_int32 * 3), ("bbox_max", C.c_int32 * 3)]
@@ -126,6 +134,18 @@ def lib():
         L.orc_render_density_threshold.argtypes = [C.c_double, C.POINTER(C.c_int)]
         L.orc_init_cells_render.restype = None
         L.orc_init_cells_render.argtypes = [p32, u32, u32, u32, u32, u32, u32, u32, u32, C.c_double]
+        vp = C.c_void_p
+        L.ofs_create.restype = vp
+        L.ofs_create.argtypes = [u32, u32, u32, p32]
+        L.ofs_destroy.restype = None
+        L.ofs_destroy.argtypes = [vp]
+        L.ofs_read.restype = None
+        L.ofs_read.argtypes = [vp, p32]
+        L.ofs_run.restype = C.c_int
+        L.ofs_run.argtypes = [vp, u32, u32, u64, u64, u32, C.c_int, C.POINTER(TickStats)]
+        L.ofs_run_sims.restype = C.c_size_t
+        L.ofs_run_sims.argtypes = [C.POINTER(Sim), C.c_size_t, C.c_int, C.c_double, p32, C.POINTER(GridInfo),
+                                   C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -260,3 +280,67 @@ def run_search(grid, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn, s
     n = lib().orc_run_search(grid, sel_flags, max_rule_bits, min_m, max_m, max_ticks, spawn,
                              spawn_mode, float(density), seed, n_threads, counts)
     return int(n), (int(counts[0]), int(counts[1]), int(counts[2]))
+
+
+# ---- the fast CPU form (cpu_fast.c): bit-sliced, threaded ----
+class FastGrid:
+    """A grid on the fast CPU form.  Same step as step(); the cells go in and
+    come out in the packed layout."""
+
+    def __init__(self, cells, X, Y, Z):
+        self.X, self.Y, self.Z = X, Y, Z
+        self._h = lib().ofs_create(X, Y, Z, _p32(np.ascontiguousarray(cells, dtype=np.uint32)))
+        if not self._h:
+            raise MemoryError("ofs_create failed")
+
+    def run(self, M, flags, birth, survival, ticks, n_threads=1, stats=True):
+        per = (TickStats * ticks)() if stats else None
+        rc = lib().ofs_run(self._h, M, flags, birth, survival, ticks, n_threads, per)
+        if rc != 0:
+            raise RuntimeError(f"ofs_run failed ({rc})")
+        return [per[i] for i in range(ticks)] if stats else None
+
+    def cells(self):
+        out = np.zeros(cell_words(self.X, self.Y, self.Z), dtype=np.uint32)
+        lib().ofs_read(self._h, _p32(out))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().ofs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+SIM_NP = np.dtype([(n, np.uint32) for n in ("X", "Y", "Z", "M", "flags", "max_ticks")] +
+                  [("birth", np.uint64), ("survival", np.uint64)] +
+                  [(n, np.uint32) for n in ("sx", "sy", "sz", "spawn_mode", "seed", "reserved")] +
+                  [("density", np.float64)])
+assert SIM_NP.itemsize == C.sizeof(Sim)
+
+
+def run_sims_fast(sims, n_threads=1, budget_s=0.0):
+    """Runs a list of simulations on the fast CPU form: `sims` is a SIM_NP
+    array or a list of dicts with its fields.  Returns (ticks per sim,
+    UINT32_MAX where not run; grid infos; number run; cell-updates executed;
+    seconds inside the call)."""
+    if not isinstance(sims, np.ndarray):
+        arr = np.zeros(len(sims), dtype=SIM_NP)
+        for i, sp in enumerate(sims):
+            for k, v in sp.items():
+                arr[k][i] = v
+        sims = arr
+    sims = np.ascontiguousarray(sims, dtype=SIM_NP)
+    n = len(sims)
+    ticks = np.zeros(max(n, 1), dtype=np.uint32)
+    infos = (GridInfo * max(n, 1))()
+    ups = C.c_uint64(0)
+    import time
+
+    t0 = time.perf_counter()
+    done = lib().ofs_run_sims(sims.ctypes.data_as(C.POINTER(Sim)), n, n_threads, float(budget_s), _p32(ticks),
+                              infos, C.byref(ups))
+    dt = time.perf_counter() - t0
+    return ticks[:n], [infos[i] for i in range(n)], int(done), int(ups.value), dt
