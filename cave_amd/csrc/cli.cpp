@@ -383,7 +383,7 @@ int main(int argc, char **argv)
         const std::string mode = a.str("mode", "");
         if (mode == "search") return run_search(a);
         if (mode == "step" || (mode == "render" && a.flag("headless"))) return run_step(a, mode == "render");
-        log_error("Unknown mode: " + mode + ". Use search or step (render / video / looking-glass are not part of this build).");
+        log_error("Unknown mode: " + mode + ". Use search, step or render --headless (the interactive render, video and looking-glass modes are not implemented here).");
         return 1;
     } catch (const ParseError &e) {
         std::fprintf(stderr, "Argument parse error: %s\n", e.what());

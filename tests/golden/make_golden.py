@@ -47,8 +47,8 @@ def spec(X, Y, Z, M, flags, birth, surv, spawn, mode, density, seed, ticks):
 def main():
     B = O.rule_string_to_bitmask("5,6")
     S = O.rule_string_to_bitmask("4,5,6")
-    cfg1 = spec(49, 49, 49, 1, 3, B, S, (16, 16, 16), 0, 0.35, 1234, 20)
-    out = {"spec": cfg1, "rows": run(49, 49, 49, 1, 3, B, S, (16, 16, 16), 0, 0.35, 1234, 20)}
+    fe49 = spec(49, 49, 49, 1, 3, B, S, (16, 16, 16), 0, 0.35, 1234, 20)
+    out = {"spec": fe49, "rows": run(49, 49, 49, 1, 3, B, S, (16, 16, 16), 0, 0.35, 1234, 20)}
     with open(os.path.join(HERE, "step_49_FE_C2.json"), "w") as f:
         json.dump(out, f, indent=0)
 

@@ -82,7 +82,7 @@ def test_global_tier_grids(engine):
     check(random_sims(6, 2, dims, ticks=6), engine)
 
 
-def test_engines_agree_on_cfg3_slice():
+def test_engines_agree_on_uniform_search_slice():
     # 49^3 FE, C in 2..5, <= 2 birth / survival bits from counts 1..18, 12^3 Bernoulli(0.3)
     vb = cv.enumerate_valid_bitmasks(18, 2)
     rng = cv.Rng(11)
