@@ -205,11 +205,6 @@ CAVE_HD void apply_rule(const Bits<5> &n, uint32_t born_lut, uint32_t surv_lut, 
     }
 }
 
-// Same tables for R rows at once.  The scalar loop over the counts present in
-// either table (a wavefront-uniform value) is shared by the rows; per count
-// and row the minterm [N == v] is a chain of five XOR-ANDs (the XOR applies
-// the count bit's polarity), and each table takes it with one AND-OR against
-// a uniform all-ones / zero mask -- seven instructions, no branches.
 // live_mask as four (a ^ b) & c steps, with `en` folded into the first.
 __device__ __forceinline__ uint32_t live_mask_v3(const uint32_t P[4], const uint32_t nm[4], uint32_t en)
 {
@@ -219,6 +214,11 @@ __device__ __forceinline__ uint32_t live_mask_v3(const uint32_t P[4], const uint
     return xor_and(P[0], nm[0], m);
 }
 
+// Same tables for R rows at once.  The scalar loop over the counts present in
+// either table (a wavefront-uniform value) is shared by the rows; per count
+// and row the minterm [N == v] is a chain of five XOR-ANDs (the XOR applies
+// the count bit's polarity), and each table takes it with one AND-OR against
+// a uniform all-ones / zero mask -- seven instructions, no branches.
 template <int R>
 __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_lut, uint32_t *tb,
                                                 uint32_t *ts)
