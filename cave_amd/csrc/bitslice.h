@@ -1,5 +1,6 @@
 // bitslice.h -- bit-sliced cell arithmetic shared by the streaming and the
-// batched kernels (host + device, so the pieces are unit-testable on a CPU).
+// batched kernels.  The layout and mask helpers are host + device; the adders
+// and the rule are built on gfx950's v_bitop3 and are device code.
 //
 // A "plane word" holds one bit of 32 cells.  Cell states (4 bits) become four
 // plane words P0..P3; a live mask is one plane word; neighbour counts are
@@ -17,7 +18,14 @@ namespace cave {
 
 template <int N>
 struct Bits {
+    static constexpr int n = N;
     uint32_t b[N];
+};
+
+// A count that is always zero (a neighbour class the rule does not use).
+template <>
+struct Bits<0> {
+    static constexpr int n = 0;
 };
 
 // ---- nibble words <-> bit planes ----
@@ -64,52 +72,119 @@ CAVE_HD uint32_t shift_right_neighbor_il(uint32_t a, uint32_t next)
 CAVE_HD uint32_t shift_left_neighbor_nat(uint32_t a, uint32_t prev) { return (a << 1) | (prev >> 31); }
 CAVE_HD uint32_t shift_right_neighbor_nat(uint32_t a, uint32_t next) { return (a >> 1) | (next << 31); }
 
-// ---- bit-sliced adders ----
-CAVE_HD void full_add(uint32_t x, uint32_t y, uint32_t z, uint32_t &s, uint32_t &c)
-{
-    const uint32_t t = x ^ y;
-    s = t ^ z;
-    c = (t & z) | (~t & x);
+// ---- gfx950 three-input bit operations ----
+// v_bitop3_b32 evaluates any function of three words in one instruction; its
+// truth table is the function applied to S0 = 0xF0, S1 = 0xCC, S2 = 0xAA.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{   // a ^ b ^ c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x96);
 }
 
-// a + b + c, W output planes (the caller guarantees the sum fits).  Column i
-// receives a_i, b_i, c_i and up to two carries of column i-1; constant-zero
-// planes fold away at compile time.
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c)
+{   // majority(a, b, c): the carry of a + b + c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xE8);
+}
+
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c)
+{   // (a ^ b) & c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x28);
+}
+
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t b, uint32_t c)
+{   // (a & b) | c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xEA);
+}
+
+// ---- bit-sliced adders ----
+// The compiler cannot simplify through v_bitop3, so the adders decide at
+// compile time which words of each column exist (planes of the operands,
+// carries of the previous column) and add only those: three words are one
+// xor3 + one maj3, two words one XOR + one AND, one word passes through.
+
+// sum and carry of the first P words of x
+template <int P>
+__device__ __forceinline__ void column_add(const uint32_t *x, uint32_t &s, uint32_t &c)
+{
+    if constexpr (P == 3) {
+        s = xor3(x[0], x[1], x[2]);
+        c = maj3(x[0], x[1], x[2]);
+    } else if constexpr (P == 2) {
+        s = x[0] ^ x[1];
+        c = x[0] & x[1];
+    } else if constexpr (P == 1) {
+        s = x[0];
+        c = 0u;
+    } else {
+        s = 0u;
+        c = 0u;
+    }
+}
+
+// columns I.. of a + b + c; k holds the NK carries out of column I - 1
+template <int W, int I, int NK, int NA, int NB, int NC>
+__device__ __forceinline__ void add3_columns(const Bits<NA> &a, const Bits<NB> &b, const Bits<NC> &c,
+                                             const uint32_t *k, Bits<W> &r)
+{
+    if constexpr (I < W) {
+        constexpr int P = (I < NA) + (I < NB) + (I < NC);
+        uint32_t x[3] = {0u, 0u, 0u};
+        int n = 0;
+        if constexpr (I < NA) x[n++] = a.b[I];
+        if constexpr (I < NB) x[n++] = b.b[I];
+        if constexpr (I < NC) x[n++] = c.b[I];
+        uint32_t s1, c1;
+        column_add<P>(x, s1, c1);
+        constexpr int P2 = (P > 0) + NK;
+        uint32_t y[3] = {0u, 0u, 0u};
+        int m = 0;
+        if constexpr (P > 0) y[m++] = s1;
+        if constexpr (NK >= 1) y[m++] = k[0];
+        if constexpr (NK >= 2) y[m++] = k[1];
+        uint32_t s2, c2;
+        column_add<P2>(y, s2, c2);
+        r.b[I] = s2;
+        uint32_t kn[2] = {0u, 0u};
+        int q = 0;
+        if constexpr (P >= 2) kn[q++] = c1;
+        if constexpr (P2 >= 2) kn[q++] = c2;
+        add3_columns<W, I + 1, (P >= 2) + (P2 >= 2)>(a, b, c, kn, r);
+    }
+}
+
+// a + b + c, W output planes (the caller guarantees the sum fits).
 template <int W, int NA, int NB, int NC>
-CAVE_HD Bits<W> add3(const Bits<NA> &a, const Bits<NB> &b, const Bits<NC> &c)
+__device__ __forceinline__ Bits<W> add3(const Bits<NA> &a, const Bits<NB> &b, const Bits<NC> &c)
 {
     Bits<W> r;
-    uint32_t k1 = 0, k2 = 0;
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-        const uint32_t ai = i < NA ? a.b[i < NA ? i : 0] : 0u;
-        const uint32_t bi = i < NB ? b.b[i < NB ? i : 0] : 0u;
-        const uint32_t ci = i < NC ? c.b[i < NC ? i : 0] : 0u;
-        uint32_t s1, c1, s2, c2;
-        full_add(ai, bi, ci, s1, c1);
-        full_add(s1, k1, k2, s2, c2);
-        r.b[i] = s2;
-        k1 = c1;
-        k2 = c2;
-    }
+    const uint32_t k[2] = {0u, 0u};
+    add3_columns<W, 0, 0>(a, b, c, k, r);
     return r;
 }
 
-// a + b, W output planes, as a ripple-carry adder with one carry per column
-// (sum = a ^ b ^ k, carry = majority(a, b, k): one three-input bit operation
-// each).  add3 with a zero third operand costs two half adders per column.
+// columns I.. of a + b; K: column I - 1 carries out
+template <int W, int I, bool K, int NA, int NB>
+__device__ __forceinline__ void add2_columns(const Bits<NA> &a, const Bits<NB> &b, uint32_t k, Bits<W> &r)
+{
+    if constexpr (I < W) {
+        constexpr int P = (I < NA) + (I < NB) + (K ? 1 : 0);
+        uint32_t x[3] = {0u, 0u, 0u};
+        int n = 0;
+        if constexpr (I < NA) x[n++] = a.b[I];
+        if constexpr (I < NB) x[n++] = b.b[I];
+        if constexpr (K) x[n++] = k;
+        uint32_t s, c;
+        column_add<P>(x, s, c);
+        r.b[I] = s;
+        add2_columns<W, I + 1, (P >= 2)>(a, b, c, r);
+    }
+}
+
+// a + b, W output planes, as a ripple-carry adder.
 template <int W, int NA, int NB>
-CAVE_HD Bits<W> add2(const Bits<NA> &a, const Bits<NB> &b)
+__device__ __forceinline__ Bits<W> add2(const Bits<NA> &a, const Bits<NB> &b)
 {
     Bits<W> r;
-    uint32_t k = 0;
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-        const uint32_t ai = i < NA ? a.b[i < NA ? i : 0] : 0u;
-        const uint32_t bi = i < NB ? b.b[i < NB ? i : 0] : 0u;
-        r.b[i] = ai ^ bi ^ k;
-        k = (ai & bi) | (k & (ai ^ bi));
-    }
+    add2_columns<W, 0, false>(a, b, 0u, r);
     return r;
 }
 
@@ -124,14 +199,14 @@ struct RowSums {
     Bits<2> xs;
 };
 
-CAVE_HD RowSums row_sums(uint32_t a, uint32_t l, uint32_t r)
-{
+__device__ __forceinline__ RowSums row_sums(uint32_t a, uint32_t l, uint32_t r)
+{   // XS is one full adder; LR is dropped by the compiler where no kind uses it
     RowSums s;
     s.a = a;
     s.lr.b[0] = l ^ r;
     s.lr.b[1] = l & r;
-    s.xs.b[0] = a ^ s.lr.b[0];
-    s.xs.b[1] = s.lr.b[1] | (a & s.lr.b[0]);
+    s.xs.b[0] = xor3(a, l, r);
+    s.xs.b[1] = maj3(a, l, r);
     return s;
 }
 
@@ -144,47 +219,36 @@ CAVE_HD RowSums row_sums(uint32_t a, uint32_t l, uint32_t r)
 // Then, per cell, with S0(z) = k1(y-1) + k0(y) + k1(y+1) and
 // S1(z) = k2(y-1) + k1(y) + k2(y+1) at plane z:
 //   N = S1(z-1) + S0(z) + S1(z+1).
+// Each kind has only the planes it can use: Bits<2> for the two-neighbour
+// sums, Bits<1> for the cell alone, Bits<0> when the classes are off.
 template <uint32_t FLAGS>
-CAVE_HD Bits<2> kind0(const RowSums &s)
+CAVE_HD auto kind0(const RowSums &s)
 {
-    Bits<2> k{{0u, 0u}};
-    if (FLAGS & 1u) k = s.lr;
-    return k;
+    if constexpr ((FLAGS & 1u) != 0u) return s.lr;
+    else return Bits<0>{};
 }
 
 template <uint32_t FLAGS>
-CAVE_HD Bits<2> kind1(const RowSums &s)
+CAVE_HD auto kind1(const RowSums &s)
 {
     constexpr bool F = FLAGS & 1u, E = FLAGS & 2u;
-    if (F && E) return s.xs;
-    if (E) return s.lr;
-    Bits<2> k{{F ? s.a : 0u, 0u}};
-    return k;
+    if constexpr (F && E) return s.xs;
+    else if constexpr (E) return s.lr;
+    else if constexpr (F) return Bits<1>{{s.a}};
+    else return Bits<0>{};
 }
 
 template <uint32_t FLAGS>
-CAVE_HD Bits<2> kind2(const RowSums &s)
+CAVE_HD auto kind2(const RowSums &s)
 {
     constexpr bool E = FLAGS & 2u, C = FLAGS & 4u;
-    if (E && C) return s.xs;
-    if (C) return s.lr;
-    Bits<2> k{{E ? s.a : 0u, 0u}};
-    return k;
+    if constexpr (E && C) return s.xs;
+    else if constexpr (C) return s.lr;
+    else if constexpr (E) return Bits<1>{{s.a}};
+    else return Bits<0>{};
 }
 
 // ---- rule ----
-// gfx950 three-input bit operations (v_bitop3_b32; truth table over S0 = 0xF0,
-// S1 = 0xCC, S2 = 0xAA): one instruction for a XOR feeding an AND, and for an
-// AND feeding an OR.
-__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c)
-{   // (a ^ b) & c
-    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x28);
-}
-
-__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t b, uint32_t c)
-{   // (a & b) | c
-    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xEA);
-}
 
 // Evaluates the born / survive tables for 32 cells at once: for every count v
 // that appears in either table, the minterm [N == v] is built from the five

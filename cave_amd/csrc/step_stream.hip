@@ -182,23 +182,32 @@ struct StreamLane {
         }
 
         // neighbour rows across slot boundaries
-        Bits<2> k1_up, k2_up, k1_dn, k2_dn;
+        using K1 = decltype(kind1<FLAGS>(rs[0]));
+        using K2 = decltype(kind2<FLAGS>(rs[0]));
+        K1 k1_up, k1_dn;
+        K2 k2_up, k2_dn;
         {
-            const Bits<2> k1_last = kind1<FLAGS>(rs[RPL - 1]), k2_last = kind2<FLAGS>(rs[RPL - 1]);
-            const Bits<2> k1_first = kind1<FLAGS>(rs[0]), k2_first = kind2<FLAGS>(rs[0]);
-            const Bits<2> k1_h = kind1<FLAGS>(rs[RPL]), k2_h = kind2<FLAGS>(rs[RPL]);
+            const K1 k1_last = kind1<FLAGS>(rs[RPL - 1]), k1_first = kind1<FLAGS>(rs[0]), k1_h = kind1<FLAGS>(rs[RPL]);
+            const K2 k2_last = kind2<FLAGS>(rs[RPL - 1]), k2_first = kind2<FLAGS>(rs[0]), k2_h = kind2<FLAGS>(rs[RPL]);
             const int src_up = ((int)lane - (int)lpr) & 63;  // slot - 1, same xg
             const int src_dn = ((int)lane + (int)lpr) & 63;  // slot + 1, same xg
+            if constexpr (K1::n > 0) {
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const uint32_t u1 = (uint32_t)__shfl((int)k1_last.b[k], src_up, 64);
-                const uint32_t u2 = (uint32_t)__shfl((int)k2_last.b[k], src_up, 64);
-                const uint32_t d1 = (uint32_t)__shfl((int)k1_first.b[k], src_dn, 64);
-                const uint32_t d2 = (uint32_t)__shfl((int)k2_first.b[k], src_dn, 64);
-                k1_up.b[k] = slot == 0 ? k1_h.b[k] : u1;
-                k2_up.b[k] = slot == 0 ? k2_h.b[k] : u2;
-                k1_dn.b[k] = slot == nslots - 1 ? k1_h.b[k] : d1;
-                k2_dn.b[k] = slot == nslots - 1 ? k2_h.b[k] : d2;
+                for (int k = 0; k < K1::n; k++) {
+                    const uint32_t u1 = (uint32_t)__shfl((int)k1_last.b[k], src_up, 64);
+                    const uint32_t d1 = (uint32_t)__shfl((int)k1_first.b[k], src_dn, 64);
+                    k1_up.b[k] = slot == 0 ? k1_h.b[k] : u1;
+                    k1_dn.b[k] = slot == nslots - 1 ? k1_h.b[k] : d1;
+                }
+            }
+            if constexpr (K2::n > 0) {
+#pragma unroll
+                for (int k = 0; k < K2::n; k++) {
+                    const uint32_t u2 = (uint32_t)__shfl((int)k2_last.b[k], src_up, 64);
+                    const uint32_t d2 = (uint32_t)__shfl((int)k2_first.b[k], src_dn, 64);
+                    k2_up.b[k] = slot == 0 ? k2_h.b[k] : u2;
+                    k2_dn.b[k] = slot == nslots - 1 ? k2_h.b[k] : d2;
+                }
             }
         }
 
@@ -206,10 +215,10 @@ struct StreamLane {
         Bits<4> S1c[RPL];
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
-            const Bits<2> u1 = r == 0 ? k1_up : kind1<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
-            const Bits<2> u2 = r == 0 ? k2_up : kind2<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
-            const Bits<2> d1 = r == RPL - 1 ? k1_dn : kind1<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
-            const Bits<2> d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
+            const K1 u1 = r == 0 ? k1_up : kind1<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
+            const K2 u2 = r == 0 ? k2_up : kind2<FLAGS>(rs[r - 1 >= 0 ? r - 1 : 0]);
+            const K1 d1 = r == RPL - 1 ? k1_dn : kind1<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
+            const K2 d2 = r == RPL - 1 ? k2_dn : kind2<FLAGS>(rs[r + 1 < RPL ? r + 1 : 0]);
             S1c[r] = add3<4>(u2, kind1<FLAGS>(rs[r]), d2);
             if constexpr (kBoxSum<FLAGS>) {
                 co.Q[r] = add2<5>(ci.S1p[r], S1c[r]);
