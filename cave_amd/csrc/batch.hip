@@ -224,8 +224,8 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
             const uint32_t set_m = vmask & ((Al & ts) | (~NZ & tb));
             const uint32_t surv = set_m & Al;
             acc.surv += __popc(surv);
-            acc.born += __popc(set_m ^ surv);
-            acc.dying += __popc(NZ & ~surv);
+            acc.born += __popc(set_m);  // born + surviving: the tick's caller takes surv out
+            acc.dying += __popc(and_not(NZ, surv));
             update_planes(pl, set_m, NZ, rc.mk);
             S.st4(poff, make_uint4(pl[0], pl[1], pl[2], pl[3]));
             st_live<COH>(S.at(noff), set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
@@ -565,6 +565,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         else if (multi) run_tick<true>(j.flags, g, SG, acur, anext, rc, acc);
         else run_tick<false>(j.flags, g, SG, acur, anext, rc, acc);
         ticks_run++;
+        acc.born -= acc.surv;  // run_item counted every cell set to M
         uint32_t tot[3];
         int32_t bmin[3], bmax[3];
         block_sum3(acc.dying, acc.born, acc.surv, sh.sum, tot);

@@ -316,22 +316,29 @@ __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_
 }
 
 // ---- state update ----
+__device__ __forceinline__ uint32_t and_not(uint32_t a, uint32_t b)
+{   // a & ~b
+    return __builtin_amdgcn_bitop3_b32(a, b, b, (unsigned char)0x30);
+}
+
 // new = M where set_m, s-1 where s != 0 and !set_m, 0 elsewhere
 // (searchSimulationCube.slang:178-211).  mk[k] = ~0 where bit k of M is set.
-CAVE_HD void update_planes(uint32_t P[4], uint32_t set_m, uint32_t nz, const uint32_t mk[4])
+// The decrement is built already masked by keep = nz & ~set_m: bit k of
+// s - 1 is P[k] XOR the borrow (~P[0] & .. & ~P[k-1]), so each masked plane
+// keep & (P[k] ^ borrow) is one three-input op.
+__device__ __forceinline__ void update_planes(uint32_t P[4], uint32_t set_m, uint32_t nz, const uint32_t mk[4])
 {
-    uint32_t bor = ~P[0];
-    uint32_t d0 = ~P[0];
-    uint32_t d1 = P[1] ^ bor;
-    bor &= ~P[1];
-    uint32_t d2 = P[2] ^ bor;
-    bor &= ~P[2];
-    uint32_t d3 = P[3] ^ bor;
-    const uint32_t keep = nz & ~set_m;
-    P[0] = (set_m & mk[0]) | (keep & d0);
-    P[1] = (set_m & mk[1]) | (keep & d1);
-    P[2] = (set_m & mk[2]) | (keep & d2);
-    P[3] = (set_m & mk[3]) | (keep & d3);
+    const uint32_t keep = and_not(nz, set_m);
+    const uint32_t e0 = and_not(keep, P[0]);
+    const uint32_t e1 = __builtin_amdgcn_bitop3_b32(keep, P[1], P[0], (unsigned char)0x90);  // keep & ~(P1 ^ P0)
+    uint32_t bor = __builtin_amdgcn_bitop3_b32(P[0], P[1], P[1], (unsigned char)0x03);     // ~(P0 | P1)
+    const uint32_t e2 = __builtin_amdgcn_bitop3_b32(keep, P[2], bor, (unsigned char)0x60);  // keep & (P2 ^ bor)
+    bor = and_not(bor, P[2]);
+    const uint32_t e3 = __builtin_amdgcn_bitop3_b32(keep, P[3], bor, (unsigned char)0x60);
+    P[0] = and_or(set_m, mk[0], e0);
+    P[1] = and_or(set_m, mk[1], e1);
+    P[2] = and_or(set_m, mk[2], e2);
+    P[3] = and_or(set_m, mk[3], e3);
 }
 
 // Per-rule constants in the form the kernels use.

@@ -109,7 +109,7 @@ struct StreamLane {
     uint32_t roff[RPL + 1];  // byte offset of the row within a z plane (kOutOfRange if none)
     uint32_t plane_bytes;
     // statistics
-    uint32_t surv = 0, born = 0, dying = 0, nrows = 0, colmask = 0;
+    uint32_t surv = 0, setm = 0, dying = 0, nrows = 0, colmask = 0;  // setm: cells set to M (born + surviving)
     uint32_t yany[RPL];
     int zmin = 0x7fffffff, zmax = -1;
 
@@ -246,8 +246,8 @@ struct StreamLane {
                 const uint32_t set_m = valid & ((A & ts[r]) | (~NZ & tb[r]));
                 const uint32_t sv = set_m & A;
                 surv += __popc(sv);
-                born += __popc(set_m ^ sv);
-                dying += __popc(NZ & ~sv);
+                setm += __popc(set_m);
+                dying += __popc(and_not(NZ, sv));
                 yany[r] |= set_m;
                 any |= set_m;
                 uint32_t P[4] = {ci.P[r][0], ci.P[r][1], ci.P[r][2], ci.P[r][3]};
@@ -341,10 +341,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
         if (t <= n + 1) L.step(zs + dz * (t - 1), dz, t <= n, t >= 2, c0, c1);
 
         acc.surv = L.surv;
-        acc.born = L.born;
+        acc.born = L.setm - L.surv;
         acc.dying = L.dying;
         // cells that stayed 0 -> 0: every valid cell not counted as dying / born / surviving
-        acc.dead = (uint32_t)__popc(L.vmask) * L.nrows - L.dying - L.born - L.surv;
+        acc.dead = (uint32_t)__popc(L.vmask) * L.nrows - L.dying - L.setm;
         if (L.colmask) {
             int lo, hi;
             mask_extent<PLANAR>(L.colmask, lo, hi);
