@@ -3,8 +3,9 @@
 // One resident persistent kernel (k_serve) runs while requests are in flight.
 // The host keeps a FIFO of submitted requests and admits them in order while
 // the HBM budget, the in-flight cap and the ring allow: admission reserves the
-// request's own region for its packed state (global-tier grids) and for its
-// final cells (when a final hash is asked for) from a budgeted arena, writes
+// request's own region for its packed state (global-tier grids) from a
+// budgeted HBM arena and, when a final hash is asked for, a region of pinned
+// host memory that the kernel writes the final cells into, writes
 // its ring entries (one per part of a split grid) into host-coherent memory
 // and publishes them by raising the ring tail.  Free blocks of the resident
 // kernel claim entries as soon as they are published, so requests that arrive
@@ -38,6 +39,8 @@ namespace {
 
 constexpr uint64_t kAlign = 256;
 constexpr uint64_t kChunkBytes = 1ull << 30;
+constexpr uint64_t kFinalChunkBytes = 64ull << 20;  // pinned host staging of final cells
+constexpr uint64_t kFinalBudget = 4ull << 30;
 constexpr uint64_t kRingCap = 1ull << 16;
 constexpr uint32_t kSyncPool = 1u << 17;
 constexpr uint32_t kThreads = 1024;
@@ -50,15 +53,25 @@ constexpr double kFallbackBlockRate = 1e10;  // cell-updates/s of one block when
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
-// HBM for resident request state: lazily allocated 1 GiB chunks (or one chunk
-// of the request's size when larger), first-fit with coalescing free lists,
-// and a byte budget over live reservations.
+// Memory for resident requests: lazily allocated chunks (or one chunk of the
+// request's size when larger), first-fit with coalescing free lists, and a
+// byte budget over live reservations.  Addresses are device addresses.
+//   * HBM (host = false): request state, 1 GiB chunks.
+//   * pinned host memory (host = true): the final cells of requests that ask
+//     for them.  The kernel writes them straight into host-coherent memory,
+//     so reporting them needs no copy: a copy engine may fall back to a copy
+//     kernel, which cannot start beside the resident kernel that fills the
+//     GPU, and waiting for it would stall the request loop until that
+//     kernel closed.
 class Arena {
 public:
-    explicit Arena(uint64_t budget) : budget_(budget) {}
+    Arena(uint64_t budget, bool host, uint64_t chunk_bytes) : budget_(budget), host_(host), chunk_bytes_(chunk_bytes) {}
     ~Arena()
     {
-        for (Chunk &c : chunks_) (void)hipFree(c.base);
+        for (Chunk &c : chunks_) {
+            if (host_) (void)hipHostFree(c.hbase);
+            else (void)hipFree(c.base);
+        }
     }
     uint64_t used() const { return used_; }
     uint64_t budget() const { return budget_; }
@@ -73,9 +86,18 @@ public:
                 used_ += bytes;
                 return CAVE_OK;
             }
-        const uint64_t size = std::max(kChunkBytes, bytes);
+        const uint64_t size = std::max(chunk_bytes_, bytes);
         Chunk c;
-        if (hipMalloc(&c.base, size) != hipSuccess) return CAVE_E_NOMEM;
+        if (host_) {
+            if (hipHostMalloc(&c.hbase, size, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+                return CAVE_E_NOMEM;
+            if (hipHostGetDevicePointer(&c.base, c.hbase, 0) != hipSuccess || !c.base) {
+                (void)hipHostFree(c.hbase);
+                return CAVE_E_NOMEM;
+            }
+        } else if (hipMalloc(&c.base, size) != hipSuccess) {
+            return CAVE_E_NOMEM;
+        }
         c.size = size;
         c.free[0] = size;
         chunks_.push_back(std::move(c));
@@ -110,9 +132,20 @@ public:
         }
     }
 
+    // host view of a device address of a pinned-host arena (nullptr if none)
+    const void *host_of(uint64_t addr) const
+    {
+        for (const Chunk &c : chunks_) {
+            const uint64_t base = (uint64_t)c.base;
+            if (addr >= base && addr < base + c.size) return (const char *)c.hbase + (addr - base);
+        }
+        return nullptr;
+    }
+
 private:
     struct Chunk {
-        void *base = nullptr;
+        void *base = nullptr;   // device address
+        void *hbase = nullptr;  // host address (pinned-host arena)
         uint64_t size = 0;
         std::map<uint64_t, uint64_t> free;  // offset -> length
     };
@@ -129,7 +162,10 @@ private:
         return false;
     }
     std::vector<Chunk> chunks_;
-    uint64_t budget_, used_ = 0;
+    uint64_t budget_;
+    bool host_;
+    uint64_t chunk_bytes_;
+    uint64_t used_ = 0;
 };
 
 struct Pending {
@@ -157,7 +193,6 @@ using namespace cave;
 struct CaveServer {
     CaveServeOptions opt{};
     hipStream_t kstream = nullptr;  // the resident kernel
-    hipStream_t cstream = nullptr;  // final-cell copies and control writes
     ServeCtl *ctl = nullptr;        // host-coherent
     DevJob *ring = nullptr;         // host-coherent
     DevResult *results = nullptr;   // host-coherent
@@ -170,7 +205,8 @@ struct CaveServer {
     ServeDev *sd = nullptr;         // device
     JobSync *sync = nullptr;        // device
     uint32_t sync_next = 0;
-    Arena *arena = nullptr;
+    Arena *arena = nullptr;        // HBM: request state
+    Arena *final_arena = nullptr;  // pinned host memory: final cells
     uint32_t lds_bytes = 0, blocks = 0, resident = 0;
     bool kernel_live = false;
     std::deque<Pending> pending;
@@ -189,14 +225,13 @@ void free_server(CaveServer *s)
 {
     if (!s) return;
     if (s->kstream) (void)hipStreamSynchronize(s->kstream);
-    if (s->cstream) (void)hipStreamSynchronize(s->cstream);
     delete s->arena;
+    delete s->final_arena;
     if (s->sync) (void)hipFree(s->sync);
     if (s->sd) (void)hipFree(s->sd);
     for (void *p : {(void *)s->ctl, (void *)s->ring, (void *)s->results, (void *)s->done})
         if (p) (void)hipHostFree(p);
     if (s->kstream) (void)hipStreamDestroy(s->kstream);
-    if (s->cstream) (void)hipStreamDestroy(s->cstream);
     delete s;
 }
 
@@ -305,7 +340,7 @@ int admit(CaveServer *s)
             f.state_bytes = p.state_bytes;
         }
         if (p.final_bytes) {
-            const int rc = s->arena->alloc(p.final_bytes, f.final_addr);
+            const int rc = s->final_arena->alloc(p.final_bytes, f.final_addr);
             if (rc) {
                 if (f.state_bytes) s->arena->release(f.region, f.state_bytes);
                 if (rc == CAVE_E_BUDGET) break;
@@ -370,14 +405,13 @@ int collect(CaveServer *s)
         c.blocks = f.parts;
         c.status = r.err ? CAVE_E_STATE : CAVE_OK;
         if (f.final_bytes && !r.err) {
-            std::vector<uint32_t> cells(f.final_bytes / 4u);
-            CAVE_HIP(hipMemcpyAsync(cells.data(), (const void *)f.final_addr, f.final_bytes, hipMemcpyDeviceToHost,
-                                    s->cstream));
-            CAVE_HIP(hipStreamSynchronize(s->cstream));
-            c.result.final_hash = fnv_ref_layout(cells.data(), f.X, f.Y, f.Z);
+            // written by the kernel before its release fence and the completion flag
+            const uint32_t *cells = static_cast<const uint32_t *>(s->final_arena->host_of(f.final_addr));
+            if (!cells) return fail(CAVE_E_STATE, "final cells outside the staging arena");
+            c.result.final_hash = fnv_ref_layout(cells, f.X, f.Y, f.Z);
         }
         if (f.state_bytes) s->arena->release(f.region, f.state_bytes);
-        if (f.final_bytes) s->arena->release(f.final_addr, f.final_bytes);
+        if (f.final_bytes) s->final_arena->release(f.final_addr, f.final_bytes);
         c.finish_s = s->now();
         s->ready.push_back(c);
         s->finished++;
@@ -414,8 +448,7 @@ int cave_serve_create(const CaveServeOptions *opt, CaveServer **out)
         free_server(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CAVE_E_DEVICE, "stream creation failed"));
     if ((rc = host_coherent((void **)&s->ctl, sizeof(ServeCtl))) ||
         (rc = host_coherent((void **)&s->ring, sizeof(DevJob) * kRingCap)) ||
@@ -435,7 +468,8 @@ int cave_serve_create(const CaveServeOptions *opt, CaveServer **out)
         hipMemsetAsync(s->sync, 0, sizeof(JobSync) * kSyncPool, s->kstream) != hipSuccess ||
         hipStreamSynchronize(s->kstream) != hipSuccess)
         return bail(fail(CAVE_E_DEVICE, "device clear failed"));
-    s->arena = new Arena(s->opt.hbm_budget_bytes);
+    s->arena = new Arena(s->opt.hbm_budget_bytes, false, kChunkBytes);
+    s->final_arena = new Arena(kFinalBudget, true, kFinalChunkBytes);
     s->lds_bytes = lds_cap(kThreads);
     (void)hipFuncSetAttribute(serve_kernel(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes);
     int per = 0;

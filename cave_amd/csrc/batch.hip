@@ -697,10 +697,9 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 // Claims the next ring entry for the block (thread 0).  Idle blocks compare
 // the claim counter with the last tail seen on the device; only the block
 // holding the poll lock reads the host's tail over the bus.  That block also
-// closes the kernel -- for every block at once -- when every published entry
-// has been claimed and the host asked to stop, or no claim happened for
-// idle_limit.  Closing only when nothing is left to claim means no part of a
-// split simulation is ever left without its partner blocks.
+// closes the kernel -- for every block at once -- when every entry seen has
+// been claimed and the host asked to stop, or no claim happened for
+// idle_limit.  Entries the host publishes after that wait for the next launch.
 __device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
 {
     ServeDev *sd = a.sd;
@@ -718,22 +717,46 @@ __device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
             }
             continue;
         }
+        // The lock holder alone raises tail_seen and alone sets closing.  Once
+        // closing is set, tail_seen never moves again and every entry below
+        // it is claimed, so no block can claim a part of a split simulation
+        // whose other parts a closing kernel would leave unclaimed (those
+        // parts would wait for partners that never start, and the kernel
+        // could not end to be relaunched).  Published tails fall on whole
+        // requests, so "every entry below tail_seen claimed" means every part
+        // of every request seen is running.
         uint32_t unlocked = 0u;
-        if (__hip_atomic_compare_exchange_strong(&sd->poll_lock, &unlocked, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+        if (__hip_atomic_compare_exchange_strong(&sd->poll_lock, &unlocked, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
+            if (__hip_atomic_load(&sd->closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(&sd->poll_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
             const uint64_t tail = __hip_atomic_load(&a.ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             const uint64_t stop = __hip_atomic_load(&a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_fetch_max(&sd->tail_seen, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t before = __hip_atomic_fetch_max(&sd->tail_seen, tail, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t seen_now = before > tail ? before : tail;
             const uint64_t now_head = __hip_atomic_load(&sd->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bool close = false;
-            if (now_head >= tail) {  // every published entry is claimed
-                const uint64_t last = __hip_atomic_load(&sd->last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                close = stop != 0u || wall_clock64() - last > a.idle_limit;
+            if (now_head >= seen_now) {  // every entry seen is claimed
+                // Idle time is a signed difference: the claim stamp may come
+                // from a block whose clock read is a little ahead of this one.
+                // A zero stamp (a fresh launch) starts the idle clock now.
+                const uint64_t now = (uint64_t)wall_clock64();
+                uint64_t last = __hip_atomic_load(&sd->last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (last == 0u) {
+                    __hip_atomic_compare_exchange_strong(&sd->last_claim, &last, now, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    close = stop != 0u;
+                } else {
+                    close = stop != 0u || (int64_t)(now - last) > (int64_t)a.idle_limit;
+                }
             }
             if (close) __hip_atomic_store(&sd->closing, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&sd->poll_lock, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sd->poll_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (close) return false;
-            if (now_head < tail) continue;
+            if (now_head < seen_now) continue;
         }
         // back off: short naps while work may be arriving, longer ones after
         if (naps++ < 16u) __builtin_amdgcn_s_sleep(4);
