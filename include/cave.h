@@ -317,7 +317,8 @@ int cave_search_destroy(CaveSearch *h);
 typedef struct CaveServer CaveServer;
 
 typedef struct CaveServeOptions {
-    uint64_t hbm_budget_bytes; /* admission budget for resident cell state + final-cell staging (0 = 200 GB) */
+    uint64_t hbm_budget_bytes; /* admission budget for resident cell state (0 = 200 GB); final cells asked
+                                  for are staged in pinned host memory (up to 4 GiB at once) */
     uint32_t max_inflight;     /* requests admitted and not yet finished (0 = 16384) */
     uint32_t split_target_us;  /* a request whose one-block run is estimated longer is split (0 = 1000) */
     uint32_t idle_exit_us;     /* the resident kernel closes after this long without a claim (0 = 50000) */
