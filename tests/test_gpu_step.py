@@ -56,6 +56,10 @@ RANDOM_CASES = [
     # (X, Y, Z, M, flags)
     (1, 1, 1, 1, 7), (2, 3, 4, 2, 1), (7, 3, 2, 3, 3), (33, 5, 4, 1, 7), (31, 6, 9, 15, 2),
     (64, 9, 7, 4, 4), (100, 17, 13, 7, 7), (65, 2, 30, 5, 5), (129, 11, 6, 2, 6), (96, 33, 5, 1, 3),
+    # rows of more than 256 cells fill a whole 16-lane DPP row: the full-row
+    # variant of the streaming kernel, every neighbourhood
+    (300, 9, 6, 3, 1), (300, 9, 6, 4, 2), (300, 9, 6, 5, 3), (300, 9, 6, 6, 4), (300, 9, 6, 2, 5),
+    (300, 9, 6, 15, 6), (300, 9, 6, 7, 7), (257, 5, 4, 1, 7), (511, 4, 3, 9, 3), (512, 6, 5, 7, 7),
 ]
 
 
