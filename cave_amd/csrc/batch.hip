@@ -725,8 +725,11 @@ __device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
         // could not end to be relaunched).  Published tails fall on whole
         // requests, so "every entry below tail_seen claimed" means every part
         // of every request seen is running.
+        // Idle blocks read the lock before trying it, so they do not keep one
+        // cache line busy with read-modify-writes while split jobs rendezvous.
         uint32_t unlocked = 0u;
-        if (__hip_atomic_compare_exchange_strong(&sd->poll_lock, &unlocked, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+        if (__hip_atomic_load(&sd->poll_lock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
+            __hip_atomic_compare_exchange_strong(&sd->poll_lock, &unlocked, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
             if (__hip_atomic_load(&sd->closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 __hip_atomic_store(&sd->poll_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
