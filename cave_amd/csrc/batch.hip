@@ -761,9 +761,11 @@ __device__ bool serve_claim(const BatchArgs &a, uint64_t &jid)
             if (close) return false;
             if (now_head < seen_now) continue;
         }
-        // back off: short naps while work may be arriving, longer ones after
+        // back off: short naps while work may be arriving, then the longest
+        // nap (~8K cycles), which keeps idle blocks' polling off the memory
+        // system that split requests rendezvous through
         if (naps++ < 16u) __builtin_amdgcn_s_sleep(4);
-        else __builtin_amdgcn_s_sleep(64);
+        else __builtin_amdgcn_s_sleep(127);
     }
 }
 
