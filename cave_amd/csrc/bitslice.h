@@ -278,11 +278,16 @@ __device__ __forceinline__ uint32_t live_mask_v3(const uint32_t P[4], const uint
     return xor_and(P[0], nm[0], m);
 }
 
+__device__ __forceinline__ uint32_t xnor_and(uint32_t a, uint32_t b, uint32_t c)
+{   // ~(a ^ b) & c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x82);
+}
+
 // Same tables for R rows at once.  The scalar loop over the counts present in
 // either table (a wavefront-uniform value) is shared by the rows; per count
-// and row the minterm [N == v] is a chain of five XOR-ANDs (the XOR applies
-// the count bit's polarity), and each table takes it with one AND-OR against
-// a uniform all-ones / zero mask -- seven instructions, no branches.
+// and row the minterm [N == v] is one XNOR and four XNOR-ANDs against the
+// count's bits, and each table takes it with one AND-OR against a uniform
+// all-ones / zero mask -- seven instructions, no branches.
 template <int R>
 __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_lut, uint32_t surv_lut, uint32_t *tb,
                                                 uint32_t *ts)
@@ -295,20 +300,21 @@ __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_
         u &= u - 1u;
         const uint32_t bm = 0u - ((born_lut >> v) & 1u);
         const uint32_t sm = 0u - ((surv_lut >> v) & 1u);
-        // inv[b] = ~0 where bit b of the count is 0.  The empty asm pins each
-        // mask to a scalar register: left to itself the compiler builds the
-        // top bit's mask as a vector select (one VALU op per count).
-        uint32_t inv[5];
+        // pol[b] = ~0 where bit b of the count is 1: a one-bit sign-extending
+        // field extract, one scalar op per bit.  The empty asm pins each mask
+        // to a scalar register: left to itself the compiler builds the top
+        // bit's mask as a vector select (one VALU op per count).
+        uint32_t pol[5];
 #pragma unroll
         for (int b = 0; b < 5; b++) {
-            inv[b] = ((v >> b) & 1u) - 1u;
-            asm volatile("" : "+s"(inv[b]));
+            pol[b] = (uint32_t)__builtin_amdgcn_sbfe((int)v, b, 1);
+            asm volatile("" : "+s"(pol[b]));
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            uint32_t mt = n[r].b[0] ^ inv[0];
+            uint32_t mt = ~(n[r].b[0] ^ pol[0]);
 #pragma unroll
-            for (int b = 1; b < 5; b++) mt = xor_and(n[r].b[b], inv[b], mt);
+            for (int b = 1; b < 5; b++) mt = xnor_and(n[r].b[b], pol[b], mt);
             tb[r] = and_or(mt, bm, tb[r]);
             ts[r] = and_or(mt, sm, ts[r]);
         }
