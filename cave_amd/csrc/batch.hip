@@ -532,6 +532,12 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     const RuleConsts rc = rule_consts_for(j.flags, j.M, j.born_lut, j.surv_lut);
     const bool multi = j.parts > 1u;
     uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
+    {   // the same in every lane: kept in scalar registers for the global tier's accesses
+        const uint64_t sa = reinterpret_cast<uint64_t>(S);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)sa);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(sa >> 32));
+        S = reinterpret_cast<uint32_t *>((uint64_t)hi << 32 | lo);
+    }
     uint32_t acur = g.alive0, anext = g.alive0 + g.aplane;
     const Words<false> SL{lds};
     const Words<true> SG{(gword *)S};  // generic -> global address space
