@@ -216,7 +216,7 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
         if (z >= z0) {
             const Bits<5> n = add2<5>(ci.Q, S1c);
             uint32_t tb, ts;
-            apply_rule_rows<1>(&n, rc.born_lut, rc.surv_lut, &tb, &ts);
+            apply_rule_consts<1>(&n, rc, &tb, &ts);
             const uint4 pq = S.ld4(poff);
             uint32_t pl[4] = {pq.x, pq.y, pq.z, pq.w};
             const uint32_t NZ = nonzero_mask(pl);

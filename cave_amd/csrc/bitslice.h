@@ -347,12 +347,23 @@ __device__ __forceinline__ void update_planes(uint32_t P[4], uint32_t set_m, uin
     P[3] = and_or(set_m, mk[3], e3);
 }
 
+// Rules whose tables hold at most kRuleSlots counts in all are evaluated from
+// masks prepared once per rule (RuleConsts::slot_*), with no scalar work per
+// step; larger rules take the count loop of apply_rule_rows.
+constexpr int kRuleSlots = 4;
+
 // Per-rule constants in the form the kernels use.
 struct RuleConsts {
     uint32_t nm[4];   // ~0 where bit k of M is 0
     uint32_t mk[4];   // ~0 where bit k of (M & 15) is 1
     uint32_t en;      // ~0 unless M == 16
     uint32_t born_lut, surv_lut;
+    // per slot: ~0 where bit b of the slot's count is 1, and whether the count
+    // is in the born / survive table; unused slots hold count 31, which no
+    // neighbour count reaches (at most 27)
+    uint32_t slot_pol[kRuleSlots][5];
+    uint32_t slot_b[kRuleSlots], slot_s[kRuleSlots];
+    uint32_t use_slots;  // 1: the tables hold at most kRuleSlots counts
 };
 
 CAVE_HD RuleConsts rule_consts(uint32_t M, uint32_t born_lut, uint32_t surv_lut)
@@ -365,7 +376,46 @@ CAVE_HD RuleConsts rule_consts(uint32_t M, uint32_t born_lut, uint32_t surv_lut)
     c.en = M <= 15u ? ~0u : 0u;
     c.born_lut = born_lut;
     c.surv_lut = surv_lut;
+    uint32_t u = born_lut | surv_lut;
+    c.use_slots = __builtin_popcount(u) <= kRuleSlots ? 1u : 0u;
+    for (int k = 0; k < kRuleSlots; k++) {
+        uint32_t v = 31u, b = 0u, s = 0u;
+        if (u) {
+            v = (uint32_t)__builtin_ctz(u);
+            u &= u - 1u;
+            b = ((born_lut >> v) & 1u) ? ~0u : 0u;
+            s = ((surv_lut >> v) & 1u) ? ~0u : 0u;
+        }
+        for (int i = 0; i < 5; i++) c.slot_pol[k][i] = ((v >> i) & 1u) ? ~0u : 0u;
+        c.slot_b[k] = b;
+        c.slot_s[k] = s;
+    }
     return c;
+}
+
+// The born / survive tables for R rows: from the prepared slots when the rule
+// has at most kRuleSlots counts (seven instructions per slot and row, no
+// scalar work), else by the count loop.
+template <int R>
+__device__ __forceinline__ void apply_rule_consts(const Bits<5> *n, const RuleConsts &rc, uint32_t *tb, uint32_t *ts)
+{
+    if (rc.use_slots) {
+#pragma unroll
+        for (int r = 0; r < R; r++) tb[r] = ts[r] = 0u;
+#pragma unroll
+        for (int k = 0; k < kRuleSlots; k++) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                uint32_t mt = ~(n[r].b[0] ^ rc.slot_pol[k][0]);
+#pragma unroll
+                for (int b = 1; b < 5; b++) mt = xnor_and(n[r].b[b], rc.slot_pol[k][b], mt);
+                tb[r] = and_or(mt, rc.slot_b[k], tb[r]);
+                ts[r] = and_or(mt, rc.slot_s[k], ts[r]);
+            }
+        }
+    } else {
+        apply_rule_rows<R>(n, rc.born_lut, rc.surv_lut, tb, ts);
+    }
 }
 
 // FEV (all 26 neighbours): the count is the 3x3x3 box sum B of live cells

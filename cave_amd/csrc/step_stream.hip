@@ -236,7 +236,7 @@ struct StreamLane {
             uint32_t tb[RPL], ts[RPL];
 #pragma unroll
             for (int r = 0; r < RPL; r++) n[r] = add2<5>(ci.Q[r], S1c[r]);
-            apply_rule_rows<RPL>(n, rc.born_lut, rc.surv_lut, tb, ts);
+            apply_rule_consts<RPL>(n, rc, tb, ts);
             uint32_t any = 0u;
 #pragma unroll
             for (int r = 0; r < RPL; r++) {
