@@ -114,12 +114,28 @@ inline uint64_t job_words(uint32_t X, uint32_t Y, uint32_t Z)
 }
 
 // Work split of one tick: columns (x-word, y) times z-chunks, about one item
-// per thread of the block when the grid allows it.
+// per thread of the block when the grid allows it.  With more columns than
+// threads, the items run in rounds, and an item of c planes takes c + 2
+// steps (two halo planes).  There, the z-chunk count minimises rounds x
+// (c + 2): a 6 x 191 x 48 part on 1024 threads takes 4 chunks (5 rounds of
+// 14 steps) rather than 1 (2 rounds of 50, the second with 122 items).
 inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads)
 {
     const uint32_t cols = ((X + 31u) / 32u) * Y;
-    uint32_t z = threads / (cols ? cols : 1u);
-    if (z < 1u) z = 1u;
+    const uint32_t T = threads ? threads : 1u;
+    if (cols > T) {
+        uint32_t best = 1u;
+        uint64_t best_cost = ~0ull;
+        for (uint32_t z = 1u; z <= Z && z <= 64u; z++) {
+            const uint32_t c = (Z + z - 1u) / z;
+            const uint64_t used = (Z + c - 1u) / c;
+            const uint64_t rounds = ((uint64_t)cols * used + T - 1u) / T;
+            const uint64_t cost = rounds * (c + 2u);
+            if (cost < best_cost) best = z, best_cost = cost;
+        }
+        return best;
+    }
+    uint32_t z = T / (cols ? cols : 1u);
     if (z > Z) z = Z;
     return z;
 }

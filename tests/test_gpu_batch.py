@@ -142,6 +142,29 @@ def test_split_simulations_match_whole():
     check(sims[1::2], cv.ENGINE_PERSISTENT)
 
 
+@pytest.mark.parametrize("threads", [256, 1024])
+def test_more_columns_than_threads(threads):
+    # grids with more (x-word, y) columns than block threads run their items
+    # in rounds over several z-chunks per column; whole and split over 2 and 5
+    # blocks, every result matches the oracle
+    if threads == 256:
+        dims = [(100, 90, 20), (70, 130, 33), (129, 80, 17)]
+    else:
+        dims = [(200, 180, 12), (161, 250, 9)]
+    sims = random_sims(2 * len(dims), 9, dims, ticks=8)
+    for s in sims[::2]:
+        s.sim_flags |= cv.SIM_NO_EARLY_EXIT
+    for parts in (1, 2, 5):
+        print(f"threads={threads} max_parts={parts}", flush=True)
+        res, rows, _ = cv.batch_run(sims, per_tick=True, block_threads=threads, max_parts=parts)
+        for i, s in enumerate(sims):
+            info, orows, final = oracle_run(s)
+            X, Y, Z = s.dims
+            assert res[i].ticks_run == len(orows), f"sim {i} parts {parts}"
+            assert [t.as_tuple() for t in rows[i]] == [t.as_tuple() for t in orows], f"sim {i} parts {parts}"
+            assert res[i].final_hash == O.canonical_hash(final, X, Y, Z), f"sim {i} parts {parts}"
+
+
 def test_split_simulations_without_rows():
     # without per-tick rows, split jobs take the one-word per-tick rendezvous
     # (surviving / born totals only; dying totals and the bounding box are
