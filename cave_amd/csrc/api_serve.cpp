@@ -274,7 +274,7 @@ void plan_request(CaveServer *s, Pending &p)
         parts = std::min({parts, s->opt.max_parts, std::max(1u, j.Z / 4u), std::max(1u, s->resident / 4u)});
     }
     j.parts = parts;
-    j.zchunks = part_zchunks(j.X, j.Y, j.Z, kThreads * ipt, parts);
+    j.zchunks = part_zchunks(j.X, j.Y, j.Z, kThreads, ipt, parts);
     p.state_bytes = in_lds ? 0 : words * 4u;
     p.final_bytes = (p.spec.flags & CAVE_SIM_FINAL_HASH) ? (uint64_t)cave_cell_words(j.X, j.Y, j.Z) * 4u : 0;
 }

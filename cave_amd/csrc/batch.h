@@ -113,13 +113,13 @@ inline uint64_t job_words(uint32_t X, uint32_t Y, uint32_t Z)
     return 4ull * W * Y * Z + 2ull * ((W + 1ull) * (Y + 2ull) * (Z + 2ull) + 1ull);
 }
 
-// Work split of one tick: columns (x-word, y) times z-chunks, about one item
-// per thread of the block when the grid allows it.  With more columns than
-// threads, the items run in rounds, and an item of c planes takes c + 2
+// Work split of one tick: columns (x-word, y) times z-chunks, about `ipt`
+// items per thread of the block when the grid allows it.  With more columns
+// than threads, the items run in rounds, and an item of c planes takes c + 2
 // steps (two halo planes).  There, the z-chunk count minimises rounds x
 // (c + 2): a 6 x 191 x 48 part on 1024 threads takes 4 chunks (5 rounds of
 // 14 steps) rather than 1 (2 rounds of 50, the second with 122 items).
-inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads)
+inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads, uint32_t ipt)
 {
     const uint32_t cols = ((X + 31u) / 32u) * Y;
     const uint32_t T = threads ? threads : 1u;
@@ -135,17 +135,18 @@ inline uint32_t job_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads
         }
         return best;
     }
-    uint32_t z = T / (cols ? cols : 1u);
+    uint32_t z = T * (ipt ? ipt : 1u) / (cols ? cols : 1u);
+    if (z < 1u) z = 1u;
     if (z > Z) z = Z;
     return z;
 }
 
-// z-chunks of one part of a job split over `parts` blocks: about one item per
-// thread of the block, times the number of parts.
-inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads, uint32_t parts)
+// z-chunks of one part of a job split over `parts` blocks, as one job of its
+// planes, times the number of parts.
+inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t threads, uint32_t ipt, uint32_t parts)
 {
     const uint32_t zp = (Z + parts - 1u) / parts;
-    return job_zchunks(X, Y, zp, threads) * parts;
+    return job_zchunks(X, Y, zp, threads, ipt) * parts;
 }
 
 const void *batch_kernel();

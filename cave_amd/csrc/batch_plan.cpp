@@ -363,7 +363,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             parts = std::max(1u, parts);
         }
         j.parts = parts;
-        j.zchunks = part_zchunks(j.X, j.Y, j.Z, threads * ipt_of[i], parts);
+        j.zchunks = part_zchunks(j.X, j.Y, j.Z, threads, ipt_of[i], parts);
         plan.ipt[i] = ipt_of[i];
         if (parts > 1) {
             j.tier = kTierOwnRegion;
