@@ -119,3 +119,62 @@ def test_cli_search_sweep(tmp_path):
     rows = [json.loads(l) for l in open(agg)]
     assert len(rows) == 71532  # every rule evaluated once at this single grid / seed
     assert sum(r["viableRuns"] for r in rows) == want["viable"]
+
+
+def test_cli_jsonl_bytes(tmp_path):
+    """The JSONL bytes of a sweep with 3 random-spawn seed runs per rule.
+
+    Per-evaluation lines: the reference writes nlohmann::json's dump() of an
+    object of unsigned integers (searchSystem.cpp:959-976; nlohmann_json
+    v3.12.0, whose object is a std::map): compact, keys in byte order, decimal
+    integers, one line each.  That is Python's compact sorted dump.
+
+    Aggregate lines (ruleAggregator.h:104-131): fixed key order, the float
+    fields as float32 quotients written by a default ostream, i.e. printf
+    "%g".  The reference's map is unordered, so lines compare as a set.
+    Each expected aggregate line is rebuilt here from the per-evaluation
+    lines."""
+    out = subprocess.run([CLI, "--mode", "search", "--shape", "cube", "--grid", "9", "--neighborhood", "FE",
+                          "--chunks-per-config", "3", "--max-rule-bits", "1", "--min-cs", "2", "--max-cs-range", "3",
+                          "--ticks", "12", "--spawn", "7", "--spawn-mode", "random", "--seed-runs", "3", "--headless",
+                          "--output", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    evals, agg_lines = [], None
+    cats = {"_viable.jsonl": 2, "_possible.jsonl": 1, "_unviable.jsonl": 0}
+    for root, _, files in os.walk(tmp_path):
+        for f in files:
+            raw = open(os.path.join(root, f), "rb").read().decode("ascii")
+            if f.endswith("_aggregate.jsonl"):
+                agg_lines = raw.splitlines(keepends=True)
+                continue
+            cat = next(c for sfx, c in cats.items() if f.endswith(sfx))
+            for line in raw.splitlines(keepends=True):
+                obj = json.loads(line)
+                assert line == json.dumps(obj, sort_keys=True, separators=(",", ":")) + "\n", line
+                assert all(type(v) is int and v >= 0 for v in obj.values())
+                evals.append((obj, cat))
+    assert evals and len(evals) % 3 == 0
+    pm = (1 << 60) - 1
+    stats = {}
+    for obj, cat in evals:
+        b, sv = obj["birthAndMaxCellStateRules"], obj["survivalAndNeighborhoodRules"]
+        key = (b & pm, sv & pm, ((b >> 60) & 0xF) + 1, sv >> 60, 0)
+        st = stats.setdefault(key, {"n": 0, 2: 0, 1: 0, 0: 0, "min": 2**32 - 1, "max": 0, "sum": 0})
+        st["n"] += 1
+        st[cat] += 1
+        t = obj["ticksSurvived"]
+        st["min"], st["max"], st["sum"] = min(st["min"], t), max(st["max"], t), st["sum"] + t
+    assert len(stats) == len(evals) // 3 and all(st["n"] == 3 for st in stats.values())
+
+    def g(num, den):
+        return "%g" % float(np.float32(num) / np.float32(den))
+
+    want = sorted(
+        f'{{"birth":{k[0]},"survival":{k[1]},"maxCellState":{k[2]},"neighborhoodFlags":{k[3]},"shapeId":{k[4]},'
+        f'"totalRuns":{st["n"]},"viableRuns":{st[2]},"possibleRuns":{st[1]},"unviableRuns":{st[0]},'
+        f'"viabilityRate":{g(st[2], st["n"])},"minTicksSurvived":{st["min"]},"maxTicksSurvived":{st["max"]},'
+        f'"avgTicksSurvived":{g(st["sum"], st["n"])}}}\n'
+        for k, st in stats.items())
+    assert agg_lines is not None and sorted(agg_lines) == want
+    # fractions occur, so the float form is exercised
+    assert any("." in l for l in want)
