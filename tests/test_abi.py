@@ -72,3 +72,35 @@ def test_splitmix64_known_values():
     assert r.next() == 0xE220A8397B1DCDAF
     assert r.next() == 0x6E789E6AA1B965F4
     assert r.next() == 0x06C45D188009454F
+
+
+def _sim(**kw):
+    base = dict(dims=(8, 8, 8), M=2, flags=cave_amd.FACES, ticks=4, birth=0b100, survival=0b1100, spawn=(4, 4, 4))
+    base.update(kw)
+    return cave_amd.Sim(**base)
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(dims=(0, 8, 8)), "dims"),
+    (dict(dims=(8, 4097, 8)), "dims"),
+    (dict(dims=(4096, 4096, 4096)), "2^32"),
+    (dict(M=0), "max_cell_state"),
+    (dict(M=17), "max_cell_state"),
+    (dict(flags=16), "neighborhood"),
+    (dict(ticks=0), "ticks"),
+    (dict(spawn_mode=cave_amd.SPAWN_BUFFER), "spawn_mode"),
+    (dict(sim_flags=1 << 20), "sim flags"),
+])
+def test_batch_rejects_bad_specs_before_device_work(kw, needle):
+    # every spec is validated before the first HIP call, so this runs without
+    # a GPU; the error names the offending spec
+    with pytest.raises(cave_amd.CaveError) as e:
+        cave_amd.batch_run([_sim(), _sim(**kw)])
+    assert e.value.code == cave_amd.CAVE_E_INVALID
+    assert "spec 1" in str(e.value) and needle in str(e.value)
+
+
+def test_batch_of_nothing_is_ok():
+    # an empty batch returns at once, before any device work
+    res, rows, _ = cave_amd.batch_run([])
+    assert len(res) == 0 and rows is None
