@@ -104,3 +104,20 @@ def test_batch_of_nothing_is_ok():
     # an empty batch returns at once, before any device work
     res, rows, _ = cave_amd.batch_run([])
     assert len(res) == 0 and rows is None
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(dims=(0, 4, 4)), "grid dimensions"),
+    (dict(dims=(65537, 1, 1)), "grid dimensions"),
+    (dict(dims=(4096, 4096, 4096)), "2^32"),
+    (dict(spawn_mode=99), "spawn_mode"),
+    (dict(spawn_mode=cave_amd.SPAWN_BUFFER), "initial_cells"),
+    (dict(kernel=99), "kernel selector"),
+])
+def test_step_rejects_bad_specs_before_device_work(kw, needle):
+    args = dict(dims=(8, 8, 8))
+    args.update(kw)
+    with pytest.raises(cave_amd.CaveError) as e:
+        cave_amd.Step(args.pop("dims"), 2, cave_amd.FACES, 0b100, 0b1100, **args)
+    assert e.value.code == cave_amd.CAVE_E_INVALID
+    assert needle in str(e.value)
