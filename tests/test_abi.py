@@ -121,3 +121,23 @@ def test_step_rejects_bad_specs_before_device_work(kw, needle):
         cave_amd.Step(args.pop("dims"), 2, cave_amd.FACES, 0b100, 0b1100, **args)
     assert e.value.code == cave_amd.CAVE_E_INVALID
     assert needle in str(e.value)
+
+
+@pytest.mark.parametrize("kw,code,needle", [
+    (dict(flags=0), cave_amd.CAVE_E_INVALID, "must include F, E or C"),
+    (dict(flags=16), cave_amd.CAVE_E_INVALID, "bad neighbourhood flags"),
+    (dict(grid=0), cave_amd.CAVE_E_INVALID, "grid dimensions"),
+    (dict(max_cs=40), cave_amd.CAVE_E_INVALID, "max cell state range"),
+    (dict(ticks=0), cave_amd.CAVE_E_INVALID, "ticks"),
+    (dict(density_index=16), cave_amd.CAVE_E_INVALID, "density index"),
+    (dict(partition_index=3, partition_count=3), cave_amd.CAVE_E_INVALID, "partition index"),
+    (dict(shape=5), cave_amd.CAVE_E_UNSUPPORTED, "shape ids"),
+])
+def test_search_rejects_bad_specs_before_device_work(kw, code, needle):
+    args = dict(grid=9, flags=cave_amd.FACES, max_rule_bits=1, min_cs=2, max_cs=2, ticks=4)
+    args.update(kw)
+    with pytest.raises(cave_amd.CaveError) as e:
+        cave_amd.Search(args.pop("grid"), args.pop("flags"), args.pop("max_rule_bits"), args.pop("min_cs"),
+                        args.pop("max_cs"), args.pop("ticks"), **args)
+    assert e.value.code == code
+    assert needle in str(e.value)
