@@ -74,6 +74,10 @@ def load_traffic(summary: str):
 
 
 def run_step_bench(steps: int, warmup: int):
+    """Times `steps` ticks after `warmup` untimed ones.  Returns the wall
+    seconds and device ms of the timed ticks, the kernel name, the cells after
+    all warmup + steps ticks and the last tick's statistics (for the parity
+    check against the CPU)."""
     import cave_amd as cv
 
     birth, surv = workload_rule()
@@ -82,11 +86,12 @@ def run_step_bench(steps: int, warmup: int):
     if warmup:
         h.run(warmup, stats=True)
     t0 = time.perf_counter()
-    _, dev_ms = h.run(steps, stats=True)
+    rows, dev_ms = h.run(steps, stats=True)
     wall = time.perf_counter() - t0
     kernel = "stream" if h.kernel == cv.KERNEL_STREAM else "gather"
+    cells = h.cells()
     h.close()
-    return wall, dev_ms, kernel
+    return wall, dev_ms, kernel, cells, (rows[-1].as_tuple() if rows else None)
 
 
 def cpu_threads() -> int:
@@ -122,12 +127,15 @@ def fast_sims(specs):
     return a
 
 
-def cpu_step_baseline(target_s: float = 10.0):
+def cpu_step_baseline(gpu_cells=None, gpu_ticks: int = 0, gpu_last=None, target_s: float = 10.0):
     """Host-CPU baseline on the same workload: the fast CPU form (bit-sliced
-    64-cell words, z slabs over threads) steps the 512^3 grid for about
-    target_s.  It is checked against the scalar oracle on the first tick of
-    a 256^3 grid with the same rule and initial density; that tick also times
-    the scalar oracle (one thread)."""
+    64-cell words, z slabs over threads) steps the 512^3 grid from the same
+    initial cells for at least gpu_ticks ticks and about target_s seconds.
+    After exactly gpu_ticks ticks its cells and last-tick statistics are
+    compared with the GPU's (the parity record).  The fast form is itself
+    checked against the scalar oracle on the first tick of a 256^3 grid with
+    the same rule and initial density; that tick also times the scalar oracle
+    (one thread)."""
     import numpy as np
 
     O = oracle_module()
@@ -145,21 +153,37 @@ def cpu_step_baseline(target_s: float = 10.0):
     cells = O.init_cells(GRID, GRID, GRID, 7, (GRID, GRID, GRID), 0, 0.2, 42)
     g = O.FastGrid(cells, GRID, GRID, GRID)
     del cells
+    parity = None
     ticks, t0 = 0, time.perf_counter()
     while True:
-        g.run(7, 7, birth, surv, 4, n_threads=n, stats=False)
-        ticks += 4
-        if time.perf_counter() - t0 >= target_s:
+        if ticks < gpu_ticks:
+            k = min(64, gpu_ticks - ticks)
+            rows = g.run(7, 7, birth, surv, k, n_threads=n, stats=ticks + k == gpu_ticks)
+            ticks += k
+            if ticks == gpu_ticks and gpu_cells is not None:
+                same_cells = bool(np.array_equal(g.cells(), gpu_cells))
+                same_stats = gpu_last is None or rows[-1].as_tuple() == tuple(gpu_last)
+                parity = {"ticks": gpu_ticks, "match": same_cells and same_stats, "cells_equal": same_cells,
+                          "last_tick_stats_equal": same_stats,
+                          "against": "fast CPU form, same initial cells and rule, after every warmup and "
+                                     "timed tick"}
+                if not parity["match"]:
+                    log(f"PARITY FAILURE: GPU and CPU grids differ after {gpu_ticks} ticks")
+        else:
+            g.run(7, 7, birth, surv, 4, n_threads=n, stats=False)
+            ticks += 4
+        if ticks >= gpu_ticks and time.perf_counter() - t0 >= target_s:
             break
     dt = time.perf_counter() - t0
     g.close()
-    return {"value": GRID ** 3 * ticks / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
+    base = {"value": GRID ** 3 * ticks / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
             "sample": f"{ticks} ticks of the 512^3 FEV C=8 workload, fast CPU form (bit-sliced 64-cell words, "
                       f"z slabs over {n} threads); checked equal to the scalar oracle on the first tick of a "
                       f"{E}^3 grid of the same rule and density",
             "scalar_oracle": {"value": E ** 3 / scalar_s, "unit": "cell-updates/s", "cores": 1,
                               "sample": f"1 tick of a {E}^3 grid of the same rule and density, scalar per-cell "
                                         "oracle"}}
+    return base, parity
 
 
 def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
@@ -452,7 +476,7 @@ def main():
 
     dev, cus, mem = cv.device_info()
     log(f"single-grid step: {a.steps} ticks of 512^3 FEV")
-    wall, dev_ms, kernel = run_step_bench(a.steps, a.warmup)
+    wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup)
     cells = GRID ** 3
     value = cells * a.steps / wall
     launch_s = dev_ms / 1e3 / a.steps
@@ -482,9 +506,11 @@ def main():
     model, ncpu = cpu_info()
     if not a.no_cpu:
         log("host-CPU baseline on the single-grid workload")
-        cb = cpu_step_baseline()
+        cb, parity = cpu_step_baseline(gpu_cells, a.warmup + a.steps, gpu_last)
         cb["host"] = f"{model} ({ncpu} logical CPUs visible)"
         out["cpu_baseline"] = cb
+        out["parity"] = parity
+    del gpu_cells
     if not a.no_search:
         log(f"rule search: {a.search_configs} configs x 32 seeds at 49^3")
         out["search"] = run_search_bench(a.search_configs, plain_configs=a.plain_configs, cpu=not a.no_cpu)

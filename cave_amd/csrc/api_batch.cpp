@@ -161,7 +161,10 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
     if ((rc = dev_alloc(b1, max_words * 4))) return rc;
     if ((rc = dev_alloc(dparts, sizeof(StatPart) * max_parts))) return rc;
     if ((rc = dev_alloc(dstate, sizeof(PlainState) * n))) return rc;
-    if (want_pt && (rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
+    if (want_pt) {  // rows past a simulation's ticks_run come back zero
+        if ((rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
+        CAVE_HIP(hipMemsetAsync(dpt.p, 0, sizeof(CaveTickStats) * n * stride, st));
+    }
     CAVE_HIP(hipMemsetAsync(dstate.p, 0, sizeof(PlainState) * n, st));
 
     hipEvent_t e0, e1;

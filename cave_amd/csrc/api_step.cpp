@@ -54,6 +54,15 @@ struct CaveStep {
 namespace {
 
 constexpr uint32_t kChunkTicks = 1024;
+// Device bytes of block partials one chunk of ticks may hold: a chunk is
+// min(kChunkTicks, kPartialBudget / (blocks per tick * sizeof(StatPart))) ticks.
+constexpr size_t kPartialBudget = 64u << 20;
+
+uint32_t chunk_ticks(uint32_t nparts)
+{
+    const size_t per_tick = (size_t)std::max<uint32_t>(nparts, 1u) * sizeof(StatPart);
+    return (uint32_t)std::clamp<size_t>(kPartialBudget / per_tick, 1, kChunkTicks);
+}
 
 int parse_rule(const CaveSimulationParameters &p, DevRule *out, bool *wrap, uint32_t *shape)
 {
@@ -212,7 +221,7 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     CK(hipMalloc(&h->cur, bytes));
     CK(hipMalloc(&h->nxt, bytes));
     CK(hipMemsetAsync(h->nxt, 0, bytes, h->stream));
-    h->chunk = kChunkTicks;
+    h->chunk = chunk_ticks(h->nparts);
     // CAVE_STEP_GRAPHS=1 replays each run of ticks as a captured graph.  Off by
     // default: the eager launches already run back to back on the device (the
     // same device time per tick at 49^3 .. 512^3), and a capture costs host

@@ -404,7 +404,10 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
     if ((rc = dev_alloc(dres, sizeof(DevResult) * n))) return rc;
     const uint32_t stride = opt ? opt->per_tick_stride : 0u;
     const bool want_pt = opt && opt->per_tick && stride;
-    if (want_pt && (rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
+    if (want_pt) {  // rows past a simulation's ticks_run come back zero
+        if ((rc = dev_alloc(dpt, sizeof(CaveTickStats) * n * stride))) return rc;
+        CAVE_HIP(hipMemsetAsync(dpt.p, 0, sizeof(CaveTickStats) * n * stride, st));
+    }
     if (plan.arena_words && (rc = dev_alloc(darena, sizeof(uint32_t) * plan.arena_words * plan.blocks))) return rc;
     if (plan.region_words && (rc = dev_alloc(dregion, sizeof(uint32_t) * plan.region_words))) return rc;
     if (plan.nsync) {

@@ -49,8 +49,8 @@ __global__ void __launch_bounds__(256) k_step_erd(const uint32_t *__restrict__ i
     __shared__ int32_t red[10 * 4];
     if (done && *done) return;  // the manager already ended this simulation
     StatAcc acc;
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < g.words) {
+    // grid-stride over the words (the launch is capped at kMaxStepBlocks)
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < g.words; w += gridDim.x * blockDim.x) {
         const uint32_t cur_word = in[w];
         uint32_t out_word = 0u;
         const bool fF = rule.flags & 1u, fE = rule.flags & 2u;
@@ -121,7 +121,10 @@ static ErdGeom erd_geom(uint32_t X, uint32_t Y, uint32_t Z)
     return g;
 }
 
-uint32_t erd_blocks(uint32_t X, uint32_t Y, uint32_t Z) { return (erd_geom(X, Y, Z).words + 255u) / 256u; }
+uint32_t erd_blocks(uint32_t X, uint32_t Y, uint32_t Z)
+{
+    return std::min<uint32_t>((erd_geom(X, Y, Z).words + 255u) / 256u, kMaxStepBlocks);
+}
 
 int launch_erd(const uint32_t *in, uint32_t *out, uint32_t X, uint32_t Y, uint32_t Z, const DevRule &rule, bool wrap,
                StatPart *partials, hipStream_t st, const uint32_t *done)

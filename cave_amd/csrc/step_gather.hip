@@ -65,9 +65,10 @@ __global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict_
     if (done && *done) return;  // the manager already ended this simulation
     StatAcc acc;
     const size_t n = (size_t)g.pitch * g.Y * g.Z;
-    const size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     const uint32_t upr = (g.X + 7u) / 8u;
-    if (w < n) {
+    // grid-stride over the words: the launch is capped at kMaxStepBlocks, so
+    // the per-tick partials stay a few hundred KiB at any grid size
+    for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < n; w += (size_t)gridDim.x * blockDim.x) {
         const uint32_t xw = (uint32_t)(w % g.pitch);
         const size_t row = w / g.pitch;
         const int y = (int)(row % g.Y);
@@ -280,7 +281,7 @@ int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st)
 uint32_t gather_blocks(const GridGeom &g)
 {
     const size_t n = (size_t)g.pitch * g.Y * g.Z;
-    return (uint32_t)((n + 255) / 256);
+    return (uint32_t)std::min<size_t>((n + 255) / 256, kMaxStepBlocks);
 }
 
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,

@@ -29,6 +29,10 @@ int launch_ref_to_planar(const uint32_t *ref, uint32_t *planar, const GridGeom &
 int launch_planar_to_ref(const uint32_t *planar, uint32_t *ref, const GridGeom &g, hipStream_t st);
 
 int launch_sanitize_nibbles(uint32_t *cells, const GridGeom &g, hipStream_t st);
+// The gather and ERD kernels are grid-stride loops over the packed words with
+// at most this many blocks (16 waves per CU on a 256-CU part), so one tick
+// leaves at most this many block partials whatever the grid size.
+constexpr uint32_t kMaxStepBlocks = 4096;
 uint32_t gather_blocks(const GridGeom &g);
 // done (nullable): device flag; when set the launch does nothing (the plain
 // per-config driver ends a simulation on the device without a host round trip)

@@ -154,7 +154,9 @@ enum {
 
 typedef struct CaveStepSpec {
     uint32_t dims[3];
-    CaveSimulationParameters params; /* raw-bitmask form (max-rule-bits 0); W is rejected */
+    CaveSimulationParameters params; /* raw-bitmask form (max-rule-bits 0).  The W flag wraps every
+                                        axis, as the render path's step does (rayMarchCompute.slang:
+                                        84-102); such handles run the wrapping gather kernel */
     uint32_t spawn[3];               /* spawn box, centred, clamped to dims */
     uint32_t spawn_mode;             /* CAVE_SPAWN_* */
     double density;                  /* random spawn probability, evaluated in double */
@@ -199,7 +201,9 @@ int cave_step_destroy(CaveStep *h);
 typedef struct CaveSimSpec {
     uint32_t dims[3];
     uint32_t max_cell_state;     /* M, 1..16 (C = M + 1 states) */
-    uint32_t neighborhood_flags; /* F|E|C; W rejected */
+    uint32_t neighborhood_flags; /* F|E|C, optionally W: cube simulations ignore W, as the cube
+                                    search kernel does (searchSimulationCube.slang:26-29); ERD
+                                    simulations (CAVE_SIM_SHAPE_ERD) wrap every axis with it */
     uint32_t ticks;              /* tick budget (MAX_NUMBER_OF_TICKS_TO_SURVIVE) */
     uint64_t birth;              /* raw bitmask, bit b = count b+1 */
     uint64_t survival;
@@ -239,7 +243,8 @@ enum {
 typedef struct CaveBatchOptions {
     uint32_t engine;         /* CAVE_ENGINE_* */
     uint32_t block_threads;  /* 0 = autotuned; receives the block size used (persistent engine) */
-    CaveTickStats *per_tick; /* nullable: n_specs * max(ticks) entries, row per spec */
+    CaveTickStats *per_tick; /* nullable: n_specs * max(ticks) entries, row per spec; entries at
+                                and after a spec's ticks_run are zero */
     uint32_t per_tick_stride; /* entries per spec row in per_tick (>= max ticks) */
     uint32_t max_parts;      /* most blocks one simulation may be split over (0 = auto, 1 = never split) */
     double device_ms;        /* out: device time of the evaluation */

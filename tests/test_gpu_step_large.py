@@ -1,0 +1,87 @@
+"""The single-grid step at the sizes the benchmark times, on the GPU vs the
+fast CPU form (oracle/cpu_fast.c, itself held bit-exact to the scalar oracle
+by tests/test_cpu_fast.py): initial cells, every tick's counters and bounding
+box, and the cells after the last tick, through the C-ABI (cave_step_*).
+
+512^3 is the benchmark grid itself: the streaming kernel's default plan there
+is 1024 blocks of 4 waves (a multiple of 8, so the XCD block remap is taken),
+64 y tiles x 64 z tiles of 8 planes, odd z tiles walking downwards, and rows
+that fill a whole 16-lane DPP row.  256^3 takes 128 blocks (also remapped)
+with two x-rows per DPP row.  200x130x70 has a partial last group and a
+partial last y and z tile.  Reference step: searchSimulationCube.slang:159-247.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cave_amd as cv
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FEV = cv.FACES | cv.EDGES | cv.CORNERS
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def bench_rule():
+    """The benchmark's rule: splitmix64 seed 2, p = 0.15 per count 1..26."""
+    rng = cv.Rng(2)
+    return rng.rule(26, 0.15), rng.rule(26, 0.15)
+
+
+def busy_rule():
+    """A rule with births and survivals around the mean live count of a
+    Bernoulli(0.2) FEV grid (26 * 0.2 = 5.2), so every bucket stays busy."""
+    return cv.rule_string_to_bitmask("4-6,9"), cv.rule_string_to_bitmask("3-7,11")
+
+
+def compare_run(dims, M, flags, birth, surv, density, seed, kernel, ticks):
+    X, Y, Z = dims
+    print(f"case {X}x{Y}x{Z} M={M} flags={flags} B={birth:#x} S={surv:#x} kernel={kernel} ticks={ticks}",
+          flush=True)
+    init = O.init_cells(X, Y, Z, M, dims, 0, density, seed)
+    fast = O.FastGrid(init, X, Y, Z)
+    with cv.Step(dims, M, flags, birth, surv, spawn=dims, spawn_mode=cv.SPAWN_RANDOM, density=density,
+                 seed=seed, kernel=kernel) as h:
+        assert h.kernel == kernel
+        assert np.array_equal(h.cells(), init), "initial cells"
+        del init
+        want = fast.run(M, flags, birth, surv, ticks, n_threads=cpu_threads(), stats=True)
+        got, _ = h.run(ticks)
+        for t in range(ticks):
+            assert got[t].as_tuple() == want[t].as_tuple(), f"tick {t}"
+        assert np.array_equal(h.cells(), fast.cells()), f"cells after {ticks} ticks"
+    fast.close()
+
+
+def test_bench_grid_512_fev_c8():
+    """The benchmark workload itself: 512^3, FEV, M = 7, its rule, Bernoulli(0.2)
+    with seed 42; 10 ticks covers the whole dying chain (6 dying states)."""
+    b, s = bench_rule()
+    compare_run((512, 512, 512), 7, FEV, b, s, 0.2, 42, cv.KERNEL_STREAM, 10)
+
+
+def test_busy_rule_512_fev_c8():
+    b, s = busy_rule()
+    compare_run((512, 512, 512), 7, FEV, b, s, 0.2, 7, cv.KERNEL_STREAM, 6)
+
+
+@pytest.mark.parametrize("kernel", [cv.KERNEL_STREAM, cv.KERNEL_STREAM_NIBBLE], ids=["stream", "stream_nibble"])
+@pytest.mark.parametrize("flags", [cv.FACES | cv.EDGES, FEV], ids=["FE", "FEV"])
+def test_256_multi_tile(kernel, flags):
+    b, s = busy_rule()
+    compare_run((256, 256, 256), 3, flags, b & 0x3FFFF, s & 0x3FFFF, 0.25, 11, kernel, 6)
+
+
+@pytest.mark.parametrize("kernel", [cv.KERNEL_STREAM, cv.KERNEL_STREAM_NIBBLE], ids=["stream", "stream_nibble"])
+def test_ragged_multi_tile(kernel):
+    b, s = busy_rule()
+    compare_run((200, 130, 70), 4, FEV, b, s, 0.3, 5, kernel, 6)
