@@ -147,6 +147,8 @@ _SIGS = {
     "cave_step_set_params": (C.c_int, [_vp, C.POINTER(SimulationParameters)]),
     "cave_step_kernel": (C.c_int, [_vp]),
     "cave_step_destroy": (C.c_int, [_vp]),
+    "cave_step_run_static": (C.c_int, [C.POINTER(SimulationParameters), C.POINTER(_u32), _u32, _vp, _vp, _u32,
+                                       _vp, _vp]),
     "cave_batch_run": (C.c_int, [C.POINTER(SimSpec), _sz, C.POINTER(SimResult), C.POINTER(BatchOptions)]),
     "cave_batch_tuning_report": (C.c_int, [C.c_char_p, _sz]),
     "cave_render_bytes": (_sz, [_u32, _u32, _u32, _u32]),
@@ -322,6 +324,18 @@ class Step:
 
     def __exit__(self, *a):
         self.close()
+
+
+def step_run_static(dims, M, flags, birth, survival, cells_a: int, cells_b: int, ticks: int,
+                    per_tick: int = 0, stream: int = 0, kernel=KERNEL_AUTO) -> None:
+    """cave_step_run_static on device pointers (integers): `ticks` ticks on
+    caller-owned buffers, no device allocation; the state ends in cells_a
+    when ticks is even, in cells_b when odd.  per_tick: device pointer to
+    `ticks` TickStats (0 = none).  Queued on `stream`, not waited for."""
+    p = encode_params(M, flags, birth, survival)
+    d = (C.c_uint32 * 3)(*dims)
+    check(lib().cave_step_run_static(C.byref(p), d, kernel, cells_a, cells_b, ticks, per_tick or None,
+                                     stream or None))
 
 
 # ---- batch ----

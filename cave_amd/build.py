@@ -41,6 +41,7 @@ LIB_SOURCES = [
     "step_gather.hip",
     "step_stream.hip",
     "step_erd.hip",
+    "api_static.hip",
 ]
 CLI_SOURCES: list[str] = ["cli.cpp"]
 

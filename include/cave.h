@@ -195,6 +195,31 @@ int cave_step_set_params(CaveStep *h, const CaveSimulationParameters *params);
 int cave_step_kernel(const CaveStep *h);
 int cave_step_destroy(CaveStep *h);
 
+/* cave_step_run_static: `ticks` ticks of the step on caller-owned device
+ * buffers, allocating nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The transition is
+ * cave_step_run's (cube shape; the W flag wraps every axis).
+ *   cells_a, cells_b: device buffers of cave_cell_words(dims) words in the
+ *     reference packed layout; cells_a holds the state before the call, with
+ *     row-padding nibbles 0.  Tick t reads one and writes the other, starting
+ *     from cells_a, so the state after the call is in cells_a when `ticks` is
+ *     even and in cells_b when it is odd.
+ *   kernel: CAVE_KERNEL_AUTO (the streaming kernel on the nibble layout when
+ *     ceil(X/8) is a multiple of 4, X <= 512, no W and a buffer below 1 GiB,
+ *     else the gather kernel), CAVE_KERNEL_GATHER or CAVE_KERNEL_STREAM_NIBBLE.
+ *   per_tick: nullable; `ticks` CaveTickStats in memory the device can write
+ *     (device memory or mapped pinned host memory).
+ *   stream: a hipStream_t (NULL: the null stream).  Work is queued on it and
+ *     the call returns without waiting.
+ * The only scratch is one tick's block partials, in a static device array of
+ * 8192 entries (320 KiB per device, held from first use until the process
+ * ends); a grid whose plan needs more is rejected with CAVE_E_INVALID.  Every
+ * call on a device shares that array, so calls that overlap on one device race. */
+int cave_step_run_static(const CaveSimulationParameters *params, const uint32_t dims[3], uint32_t kernel,
+                         uint32_t *cells_a, uint32_t *cells_b, uint32_t ticks, CaveTickStats *per_tick,
+                         void *stream);
+
 /* ================================================================== */
 /* Batched evaluation of explicit configs                              */
 /* ================================================================== */
