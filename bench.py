@@ -73,7 +73,7 @@ def load_traffic(summary: str):
     return None
 
 
-def run_step_bench(steps: int, warmup: int):
+def run_step_bench(steps: int, warmup: int, dims=(GRID, GRID, GRID)):
     """Times `steps` ticks after `warmup` untimed ones.  Returns the wall
     seconds and device ms of the timed ticks, the kernel name, the cells after
     all warmup + steps ticks and the last tick's statistics (for the parity
@@ -81,8 +81,8 @@ def run_step_bench(steps: int, warmup: int):
     import cave_amd as cv
 
     birth, surv = workload_rule()
-    h = cv.Step((GRID, GRID, GRID), 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv,
-                spawn=(GRID, GRID, GRID), spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42)
+    h = cv.Step(dims, 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv,
+                spawn=dims, spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42)
     if warmup:
         h.run(warmup, stats=True)
     t0 = time.perf_counter()
@@ -92,6 +92,32 @@ def run_step_bench(steps: int, warmup: int):
     cells = h.cells()
     h.close()
     return wall, dev_ms, kernel, cells, (rows[-1].as_tuple() if rows else None)
+
+
+def run_step_variant(dims, kernel_name: str, steps: int, warmup: int):
+    """Event-timed launches of the step workload's rule on another grid or
+    kernel form: the grid beyond the Infinity Cache (512 x 512 x 2048, a
+    512 MiB ping-pong pair against a 256 MB cache) and the nibble-layout form
+    of the streaming kernel at 512^3.  Returns the average launch time and
+    its algorithmic HBM rate (1 B per cell-update)."""
+    import cave_amd as cv
+
+    birth, surv = workload_rule()
+    kernel = {"stream": cv.KERNEL_STREAM, "stream_nibble": cv.KERNEL_STREAM_NIBBLE}[kernel_name]
+    with cv.Step(dims, 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv, spawn=dims,
+                 spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42, kernel=kernel) as h:
+        h.run(warmup, stats=False)
+        t0 = time.perf_counter()
+        _, dev_ms = h.run(steps, stats=True)
+        wall = time.perf_counter() - t0
+    cells = dims[0] * dims[1] * dims[2]
+    launch_s = dev_ms / 1e3 / steps
+    gbs = cells / launch_s / 1e9
+    summary = {((GRID, GRID, 4 * GRID), "stream"): "step_stream_512x512x2048_FEV_C8.md"}.get((tuple(dims), kernel_name))
+    return {"grid": list(dims), "kernel": f"step_{kernel_name}", "ticks": steps, "warmup": warmup,
+            "traffic": load_traffic(summary) if summary else None,
+            "avg_launch_us": launch_s * 1e6, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "cell_updates_per_s": cells * steps / wall, "state_bytes_ping_pong": cells}
 
 
 def cpu_threads() -> int:
@@ -313,10 +339,8 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max
     a grid of edge 32..192 (log-uniform), F/FE/FEV, C in 2..16, bits p = 0.12,
     20..500 ticks, admitted against a 200 GB HBM budget into the resident
     persistent kernel.  Latency = completion seen by the host - submission.
-    1% of the requests (every 100th) carry a final-cell hash and are checked
-    against the CPU oracle, cheapest first, within a bounded CPU budget."""
-    import concurrent.futures as cf
-
+    1% of the requests (every 100th) carry a final-cell hash; each one is
+    rerun on the fast CPU form and compared."""
     import numpy as np
 
     import cave_amd as cv
@@ -369,51 +393,33 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max
     }
     if cpu:
         O = oracle_module()
-
-        def check(i):
-            sp = specs[i]
-            X, Y, Z = (int(v) for v in sp["dims"])
-            init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
-                                float(sp["density"]), int(sp["seed"]))
-            t, info, _, final = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
-                                           int(sp["survival"]), int(sp["ticks"]), init)
-            r = done[i].result
-            same = (r.ticks_run == t and r.info.finalAliveCount == info.finalAliveCount and
-                    r.info.totalNumberOfCellsThatDied == info.totalNumberOfCellsThatDied and
-                    r.info.totalNumberOfCellsThatWereBorn == info.totalNumberOfCellsThatWereBorn and
-                    r.final_hash == O.canonical_hash(final, X, Y, Z))
-            return same, t * X * Y * Z
-
-        workers = min(16, os.cpu_count() or 1)
-        order = sample[np.argsort(ticks_run[sample] * cells[sample], kind="stable")]
-        # a request above 2e8 executed cell-updates takes the scalar oracle > 15 s
-        eligible = [int(i) for i in order if ticks_run[i] * cells[i] <= 2e8]
-        checked, mism, cpu_ups = 0, 0, 0
+        workers = cpu_threads()
+        # every sampled request, costliest first so the threads finish together
+        order = sample[np.argsort(-(ticks_run[sample] * cells[sample]), kind="stable")]
         t1 = time.perf_counter()
-        with cf.ThreadPoolExecutor(workers) as ex:
-            it = iter(eligible)
-            running = set()
-            for i in it:
-                running.add(ex.submit(check, i))
-                if len(running) >= workers:
-                    break
-            while running:
-                fin, running = cf.wait(running, return_when=cf.FIRST_COMPLETED)
-                for f in fin:
-                    same, u = f.result()
-                    checked += 1
-                    mism += 0 if same else 1
-                    cpu_ups += u
-                    if time.perf_counter() - t1 < 20.0:
-                        i = next(it, None)
-                        if i is not None:
-                            running.add(ex.submit(check, i))
-        out["spot_checks"] = {"sampled": int(len(sample)), "checked": checked, "mismatches": mism,
-                              "method": "every 100th request carries a final-cell hash; the CPU oracle reruns "
-                                        f"the sampled requests cheapest first for 20 s on {workers} threads "
-                                        "(requests above 2e8 executed cell-updates are not rerun)",
-                              "cpu_cell_updates_per_s": cpu_ups / (time.perf_counter() - t1), "cores": workers}
-        assert mism == 0, f"{mism} served results differ from the CPU oracle"
+        cticks, cinfo, ran, cpu_ups, _, chash = O.run_sims_fast(fast_sims(specs[order]), n_threads=workers,
+                                                              hashes=True)
+        dt = time.perf_counter() - t1
+        mism = []
+        for k, i in enumerate(order):
+            r = done[int(i)].result
+            same = (cticks[k] == r.ticks_run and cinfo[k].finalAliveCount == r.info.finalAliveCount and
+                    cinfo[k].maxAliveCount == r.info.maxAliveCount and
+                    cinfo[k].totalNumberOfCellsThatDied == r.info.totalNumberOfCellsThatDied and
+                    cinfo[k].totalNumberOfCellsThatWereBorn == r.info.totalNumberOfCellsThatWereBorn and
+                    cinfo[k].status == r.info.status and int(chash[k]) == r.final_hash)
+            if not same:
+                mism.append(int(i))
+        out["spot_checks"] = {"sampled": int(len(sample)), "checked": int(ran), "mismatches": len(mism),
+                              "split_requests_checked": int((blocks[order] > 1).sum()),
+                              "largest_checked_cell_updates": int((ticks_run[order] * cells[order]).max()),
+                              "method": "every 100th request carries a final-cell hash; every one of them is rerun "
+                                        f"on the fast CPU form (one request per thread, {workers} threads) and its "
+                                        "ticks run, grid info and final-cell hash are compared",
+                              "cpu_cell_updates_per_s": cpu_ups / dt, "cpu_seconds": dt, "cores": workers}
+        if mism:
+            log(f"SPOT-CHECK FAILURE: requests {mism[:10]} differ from the CPU")
+        assert not mism, f"{len(mism)} served results differ from the CPU"
     return out
 
 
@@ -451,6 +457,11 @@ def main():
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-configs", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--step-dims", default=f"{GRID},{GRID},{GRID}",
+                    help="grid of the single-grid step (X,Y,Z); other than 512^3 only with --no-cpu, "
+                         "for profiling one size on its own")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the 512x512x2048 and nibble-layout step timings")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--plain-configs", type=int, default=64)
     ap.add_argument("--serve-split-us", type=int, default=0,
@@ -475,13 +486,16 @@ def main():
     import cave_amd as cv
 
     dev, cus, mem = cv.device_info()
-    log(f"single-grid step: {a.steps} ticks of 512^3 FEV")
-    wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup)
-    cells = GRID ** 3
+    dims = tuple(int(v) for v in a.step_dims.split(","))
+    if dims != (GRID, GRID, GRID) and not a.no_cpu:
+        ap.error("--step-dims other than 512,512,512 needs --no-cpu (the CPU parity check runs the 512^3 grid)")
+    log(f"single-grid step: {a.steps} ticks of {dims[0]}x{dims[1]}x{dims[2]} FEV")
+    wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup, dims)
+    cells = dims[0] * dims[1] * dims[2]
     value = cells * a.steps / wall
     launch_s = dev_ms / 1e3 / a.steps
     achieved = cells * 1.0 / launch_s / 1e9  # 0.5 B read + 0.5 B write per cell-update
-    traffic = load_traffic("step_stream_512_FEV_C8.md") if kernel == "stream" else None
+    traffic = load_traffic("step_stream_512_FEV_C8.md") if kernel == "stream" and dims == (GRID,) * 3 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -495,14 +509,19 @@ def main():
         "vs_baseline": None,
         "dtype": "u4 (packed nibbles, integer)",
         "data": "synthetic (seeded cell hash + splitmix64 rules)",
-        "config": {"workload": "single grid 512^3, FEV, C=8, random B/S p=0.15, Bernoulli(0.2) init",
-                   "grid": [GRID, GRID, GRID], "neighborhood": "FEV", "cell_states": 8,
+        "config": {"workload": f"single grid {dims[0]}x{dims[1]}x{dims[2]}, FEV, C=8, random B/S p=0.15, "
+                               "Bernoulli(0.2) init",
+                   "grid": list(dims), "neighborhood": "FEV", "cell_states": 8,
                    "kernel": kernel, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"step_{kernel}", "avg_launch_us": launch_s * 1e6},
         "device": {"name": dev, "cus": cus, "hbm_gib": round(mem / 2**30, 1)},
     }
+    if not a.no_variants:
+        log("step variants: 512x512x2048 (beyond the Infinity Cache), nibble layout at 512^3")
+        out["step_beyond_cache"] = run_step_variant((GRID, GRID, 4 * GRID), "stream", 200, 200)
+        out["step_nibble"] = run_step_variant((GRID, GRID, GRID), "stream_nibble", 500, 500)
     model, ncpu = cpu_info()
     if not a.no_cpu:
         log("host-CPU baseline on the single-grid workload")

@@ -506,7 +506,7 @@ static uint32_t manager(orc_grid_info *info, const orc_tick_stats *st, uint32_t 
     return info->status;
 }
 
-static uint32_t run_sim(const ofs_sim *sp, orc_grid_info *info)
+static uint32_t run_sim(const ofs_sim *sp, orc_grid_info *info, uint64_t *hash)
 {
     memset(info, 0, sizeof *info);
     rule_t r;
@@ -531,6 +531,13 @@ static uint32_t run_sim(const ofs_sim *sp, orc_grid_info *info)
         if (manager(info, &st, sp->max_ticks) > 0 || t >= 0x7fffffffu) break;
     }
     scratch_free(&s);
+    if (hash) {  /* FNV-1a-64 of the final cells, as orc_canonical_hash */
+        uint32_t *fin = (uint32_t *)calloc(orc_cell_words(sp->X, sp->Y, sp->Z), sizeof(uint32_t));
+        if (!fin) { ofs_destroy(g); return UINT32_MAX; }
+        ofs_read(g, fin);
+        *hash = orc_canonical_hash(fin, sp->X, sp->Y, sp->Z);
+        free(fin);
+    }
     ofs_destroy(g);
     return t;
 }
@@ -545,6 +552,7 @@ typedef struct {
     double budget_s;
     uint32_t *ticks_out;
     orc_grid_info *info_out;
+    uint64_t *hash_out;
 } sims_run;
 
 static double since(const struct timespec *t0)
@@ -562,7 +570,7 @@ static void *sims_worker(void *p)
         size_t i = atomic_fetch_add(&c->next, 1);
         if (i >= c->n) break;
         orc_grid_info info;
-        uint32_t t = run_sim(&c->sims[i], &info);
+        uint32_t t = run_sim(&c->sims[i], &info, c->hash_out ? &c->hash_out[i] : NULL);
         c->ticks_out[i] = t;
         if (c->info_out) c->info_out[i] = info;
         if (t != UINT32_MAX) {
@@ -576,6 +584,12 @@ static void *sims_worker(void *p)
 size_t ofs_run_sims(const ofs_sim *sims, size_t n, int n_threads, double budget_s, uint32_t *ticks_out,
                     orc_grid_info *info_out, uint64_t *cell_updates)
 {
+    return ofs_run_sims_hash(sims, n, n_threads, budget_s, ticks_out, info_out, cell_updates, NULL);
+}
+
+size_t ofs_run_sims_hash(const ofs_sim *sims, size_t n, int n_threads, double budget_s, uint32_t *ticks_out,
+                         orc_grid_info *info_out, uint64_t *cell_updates, uint64_t *hash_out)
+{
     sims_run c;
     memset(&c, 0, sizeof c);
     c.sims = sims;
@@ -583,6 +597,7 @@ size_t ofs_run_sims(const ofs_sim *sims, size_t n, int n_threads, double budget_
     c.budget_s = budget_s;
     c.ticks_out = ticks_out;
     c.info_out = info_out;
+    c.hash_out = hash_out;
     atomic_init(&c.next, 0);
     atomic_init(&c.done, 0);
     atomic_init(&c.updates, 0);

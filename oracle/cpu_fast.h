@@ -65,6 +65,10 @@ typedef struct {
  * the cells x ticks they executed. */
 size_t ofs_run_sims(const ofs_sim *sims, size_t n, int n_threads, double budget_s, uint32_t *ticks_out,
                     orc_grid_info *info_out, uint64_t *cell_updates);
+/* ofs_run_sims, and hash_out (nullable) receives each simulation's final
+ * cells hashed as orc_canonical_hash does. */
+size_t ofs_run_sims_hash(const ofs_sim *sims, size_t n, int n_threads, double budget_s, uint32_t *ticks_out,
+                         orc_grid_info *info_out, uint64_t *cell_updates, uint64_t *hash_out);
 
 #ifdef __cplusplus
 }

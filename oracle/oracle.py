@@ -146,6 +146,9 @@ def lib():
         L.ofs_run_sims.restype = C.c_size_t
         L.ofs_run_sims.argtypes = [C.POINTER(Sim), C.c_size_t, C.c_int, C.c_double, p32, C.POINTER(GridInfo),
                                    C.POINTER(C.c_uint64)]
+        L.ofs_run_sims_hash.restype = C.c_size_t
+        L.ofs_run_sims_hash.argtypes = [C.POINTER(Sim), C.c_size_t, C.c_int, C.c_double, p32, C.POINTER(GridInfo),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -321,11 +324,12 @@ SIM_NP = np.dtype([(n, np.uint32) for n in ("X", "Y", "Z", "M", "flags", "max_ti
 assert SIM_NP.itemsize == C.sizeof(Sim)
 
 
-def run_sims_fast(sims, n_threads=1, budget_s=0.0):
+def run_sims_fast(sims, n_threads=1, budget_s=0.0, hashes=False):
     """Runs a list of simulations on the fast CPU form: `sims` is a SIM_NP
     array or a list of dicts with its fields.  Returns (ticks per sim,
     UINT32_MAX where not run; grid infos; number run; cell-updates executed;
-    seconds inside the call)."""
+    seconds inside the call), and with hashes=True a sixth item: the final
+    cells' canonical hash per simulation."""
     if not isinstance(sims, np.ndarray):
         arr = np.zeros(len(sims), dtype=SIM_NP)
         for i, sp in enumerate(sims):
@@ -339,8 +343,11 @@ def run_sims_fast(sims, n_threads=1, budget_s=0.0):
     ups = C.c_uint64(0)
     import time
 
+    hs = np.zeros(max(n, 1), dtype=np.uint64) if hashes else None
     t0 = time.perf_counter()
-    done = lib().ofs_run_sims(sims.ctypes.data_as(C.POINTER(Sim)), n, n_threads, float(budget_s), _p32(ticks),
-                              infos, C.byref(ups))
+    done = lib().ofs_run_sims_hash(sims.ctypes.data_as(C.POINTER(Sim)), n, n_threads, float(budget_s), _p32(ticks),
+                                   infos, C.byref(ups),
+                                   hs.ctypes.data_as(C.POINTER(C.c_uint64)) if hashes else None)
     dt = time.perf_counter() - t0
-    return ticks[:n], [infos[i] for i in range(n)], int(done), int(ups.value), dt
+    out = (ticks[:n], [infos[i] for i in range(n)], int(done), int(ups.value), dt)
+    return out + (hs[:n],) if hashes else out

@@ -105,15 +105,16 @@ def test_run_sims_matches_run_rule():
     """Whole simulations (fresh cells, manager after every tick, early
     death) against orc_run_rule: ticks run and every grid-info field."""
     sims = _sims(40, 11)
-    ticks, infos, done, ups, _ = O.run_sims_fast(sims, n_threads=4)
+    ticks, infos, done, ups, _, hashes = O.run_sims_fast(sims, n_threads=4, hashes=True)
     assert done == len(sims)
     total = 0
     for i, sp in enumerate(sims):
         init = O.init_cells(sp["X"], sp["Y"], sp["Z"], sp["M"], (sp["sx"], sp["sy"], sp["sz"]), 0,
                             sp["density"], sp["seed"])
-        t, info, _, _ = O.run_rule(sp["X"], sp["Y"], sp["Z"], sp["M"], sp["flags"], sp["birth"], sp["survival"],
-                                   sp["max_ticks"], init)
+        t, info, _, final = O.run_rule(sp["X"], sp["Y"], sp["Z"], sp["M"], sp["flags"], sp["birth"],
+                                       sp["survival"], sp["max_ticks"], init)
         assert ticks[i] == t, i
+        assert int(hashes[i]) == O.canonical_hash(final, sp["X"], sp["Y"], sp["Z"]), i
         for f, _ in O.GridInfo._fields_:
             assert getattr(infos[i], f) == getattr(info, f), (i, f)
         total += t * sp["X"] * sp["Y"] * sp["Z"]

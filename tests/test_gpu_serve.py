@@ -122,3 +122,36 @@ def test_relaunch_after_idle_close():
     got = {**first, **srv2}
     for i, r in enumerate(ref):
         assert key(got[i].result) == key(r), f"request {i}"
+
+
+def test_cfg5_envelope_against_the_cpu():
+    """The serving workload's own envelope: the 24 costliest requests of a 2 s
+    stream of it (edges 176..191, F/FE/FEV, C in 2..16, 374..498-tick budgets,
+    about half of them alive to the end), served together so the large ones
+    split over many blocks, each compared with the fast CPU form (ticks, every
+    grid-info field, final-cell hash)."""
+    from cave_amd import workloads as W
+    from cpu_ref import cpu_threads, fast_sims
+
+    _, specs = W.serving_requests(2000.0, 2.0)
+    cost = specs["dims"][:, 0].astype(np.int64) ** 3 * specs["ticks"]
+    idx = np.argsort(-cost, kind="stable")[:24]
+    sub = np.ascontiguousarray(specs[idx])
+    sub["sim_flags"] |= cv.SIM_FINAL_HASH
+    cticks, cinfo, ran, _, _, chash = O.run_sims_fast(fast_sims(sub), n_threads=cpu_threads(), hashes=True)
+    assert ran == len(sub) and int(cticks.max()) > 400
+    print(f"serving {len(sub)} envelope requests", flush=True)
+    with cv.Server() as srv:
+        for k in range(len(sub)):
+            srv.submit_row(sub, k, k)
+        got = {c.request_id: c for c in srv.drain()}
+    assert sorted(got) == list(range(len(sub)))
+    assert sum(c.blocks > 1 for c in got.values()) >= len(sub) // 2
+    for k in range(len(sub)):
+        c = got[k]
+        r = c.result
+        assert c.status == 0, f"request {k}"
+        assert r.ticks_run == cticks[k], f"request {k}"
+        for f, _ in O.GridInfo._fields_:
+            assert getattr(r.info, f) == getattr(cinfo[k], f), (k, f)
+        assert r.final_hash == int(chash[k]), f"request {k} blocks {c.blocks}"

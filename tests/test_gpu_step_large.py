@@ -10,25 +10,16 @@ that fill a whole 16-lane DPP row.  256^3 takes 128 blocks (also remapped)
 with two x-rows per DPP row.  200x130x70 has a partial last group and a
 partial last y and z tile.  Reference step: searchSimulationCube.slang:159-247.
 """
-import os
-
 import numpy as np
 import pytest
 
 import cave_amd as cv
 import oracle as O
+from cpu_ref import cpu_threads
 
 pytestmark = pytest.mark.gpu
 
 FEV = cv.FACES | cv.EDGES | cv.CORNERS
-
-
-def cpu_threads() -> int:
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
 
 
 def bench_rule():
