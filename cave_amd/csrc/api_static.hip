@@ -98,7 +98,7 @@ extern "C" int cave_step_run_static(const CaveSimulationParameters *params, cons
     if (kernel == CAVE_KERNEL_STREAM_NIBBLE && !use_stream) {
         if (!stream_fits)
             return fail(CAVE_E_UNSUPPORTED, "CAVE_KERNEL_STREAM_NIBBLE needs ceil(X/8) to be a multiple of 4, "
-                                            "X <= 512, no W flag and a buffer below 1 GiB");
+                                            "X <= 2048 and no W flag");
         return fail(CAVE_E_INVALID, "the streaming plan of this grid needs " + std::to_string(plan.blocks) +
                                         " block partials, above the static scratch of " +
                                         std::to_string(kScratchParts) + "; use CAVE_KERNEL_GATHER");

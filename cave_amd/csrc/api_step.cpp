@@ -157,8 +157,6 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
         if (spec->dims[a] == 0 || spec->dims[a] > 65536)
             return fail(CAVE_E_INVALID, "grid dimensions must be in 1..65536");
     const uint64_t cells = (uint64_t)spec->dims[0] * spec->dims[1] * spec->dims[2];
-    if (cells > 0xFFFFFFFFull)
-        return fail(CAVE_E_INVALID, "grid larger than 2^32 cells: per-tick counters are u32");
     if (spec->spawn_mode > CAVE_SPAWN_RENDER_RANDOM) return fail(CAVE_E_INVALID, "bad spawn_mode");
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER && !spec->initial_cells)
         return fail(CAVE_E_INVALID, "CAVE_SPAWN_BUFFER needs initial_cells");
@@ -168,6 +166,10 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     if (!h) return fail(CAVE_E_NOMEM, "host allocation failed");
     int rc = parse_rule(spec->params, &h->rule, &h->wrap, &h->shape);
     if (rc) { release(h); return rc; }
+    if (h->shape == 1u && cells > 0xFFFFFFFFull) {  // the ERD kernel indexes cells with 32 bits
+        release(h);
+        return fail(CAVE_E_INVALID, "ERD grids are limited to 2^32 - 1 cells");
+    }
     if ((h->shape == 1u) != (spec->kernel == CAVE_KERNEL_ERD) && spec->kernel != CAVE_KERNEL_AUTO) {
         release(h);
         return fail(CAVE_E_INVALID, "CAVE_KERNEL_ERD runs exactly the ERD shape (id 1)");
@@ -184,7 +186,7 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     if (want_stream && !stream_ok) {
         release(h);
         return fail(CAVE_E_UNSUPPORTED, h->wrap ? "wrapping grids run on the gather kernel"
-                                                : "the streaming kernel needs X <= 512");
+                                                : "the streaming kernel needs X <= 2048");
     }
     if (h->shape == 1u) h->kernel = CAVE_KERNEL_ERD;
     else if (spec->kernel == CAVE_KERNEL_GATHER || !stream_ok) h->kernel = CAVE_KERNEL_GATHER;

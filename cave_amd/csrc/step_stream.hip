@@ -12,12 +12,17 @@
 // Work decomposition (one tick per launch):
 //   * a lane owns a 32-cell x-group (one 128-bit load / store) of RPL
 //     consecutive y-rows, and walks a z-range of TZ planes;
-//   * LPR lanes cover one x-row, so an x-row of up to 512 cells sits in one
-//     16-lane DPP row: the x-neighbour live bits cross lanes with row_shr /
-//     row_shl DPP moves;
+//   * LPR lanes cover one x-row.  An x-row of up to 512 cells sits in one
+//     16-lane DPP row and its x-neighbour live bits cross lanes with row_shr /
+//     row_shl DPP moves; rows of 513..2048 cells take 32 or 64 lanes, and the
+//     bits cross with whole-wave wave_shr / wave_shl DPP moves;
 //   * the (64 / LPR) row slots of a wave are consecutive y blocks; the rows
 //     just above / below a slot come from the neighbouring slot with a lane
-//     shuffle, and only the wave's two edge slots load a halo row;
+//     shuffle, and only the wave's two edge slots load a halo row (a wave of
+//     one slot loads both);
+//   * each wave addresses its z slab (its planes plus one halo plane per
+//     side) through buffer descriptors based at the slab, so 32-bit offsets
+//     reach every plane whatever the grid's size;
 //   * along z the kernel keeps the per-row partial sums S0 / S1 of the last
 //     planes in registers, so each group is loaded once per tick (plus the
 //     y / z halo of the wave tile) and stored once.  The z loop is unrolled by
@@ -54,6 +59,25 @@ __device__ __forceinline__ uint32_t dpp_from_next_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, true);
 }
 
+__device__ __forceinline__ uint32_t dpp_wave_from_prev_lane(uint32_t v)
+{   // wave_shr:1 -- lane i receives lane i-1 of the wave (0 at lane 0)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ uint32_t dpp_wave_from_next_lane(uint32_t v)
+{   // wave_shl:1 -- lane i receives lane i+1 of the wave (0 at lane 63)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
+}
+
+// How a lane reaches the groups next to its own along x.
+enum : int {
+    kXRows = 0,     // several x-rows per 16-lane DPP row: row shifts, masked at the x-row ends
+    kXFullRow = 1,  // one x-row per 16-lane DPP row: row shifts, whose zero fill masks the ends
+    kXWave32 = 2,   // one x-row per 32 lanes: whole-wave shifts, masked at the x-row ends
+    kXWave64 = 3,   // one x-row per wave: whole-wave shifts, zero-filled at lanes 0 / 63; the wave
+                    // holds one row slot, whose lanes load both y halo rows
+};
+
 // Lowest / highest cell index (0..31) set in a mask.
 template <bool PLANAR>
 __device__ __forceinline__ void mask_extent(uint32_t m, int &lo, int &hi)
@@ -76,15 +100,15 @@ __device__ __forceinline__ void mask_extent(uint32_t m, int &lo, int &hi)
 }
 
 constexpr uint32_t kOutOfRange = 0x80000000u;  // any row offset this large is outside every buffer
-// Plane offset of a load that must read nothing: buffers stay below 2^30 bytes
-// (stream_supported), and a row offset (< 2^30, or kOutOfRange) plus 2^30
-// neither wraps nor lands inside one.
+// Plane offset of a load that must read nothing: a wave's slab descriptor
+// covers less than 2^30 bytes (stream_plan caps the z tile), and a row offset
+// (< 2^30, or kOutOfRange) plus 2^30 neither wraps nor lands inside it.
 constexpr uint32_t kNoPlane = 0x40000000u;
 constexpr uint32_t kBufferDword3 = 0x00020000u;  // raw buffer resource word 3 on gfx9-family parts
 
-template <int RPL>
+template <int RPL, int NR>
 struct Carry {
-    uint4 w[RPL + 1];      // raw groups of the plane the step receiving this set consumes
+    uint4 w[NR];           // raw groups of the plane the step receiving this set consumes
     uint32_t P[RPL][4];    // state planes of the last input plane
     uint32_t A[RPL];       // its live masks
     uint32_t NZ[RPL];      // its non-zero masks
@@ -92,21 +116,23 @@ struct Carry {
     Bits<5> Q[RPL];        // S1 of the plane before it + S0 of the last input plane
 };
 
-// FULLROW: an x-row takes a whole 16-lane DPP row (16 lanes per row), so the
-// DPP moves' zero fill at the row ends already masks the x-neighbours.
-template <uint32_t FLAGS, int RPL, bool PLANAR, bool FULLROW>
+// XMODE: one of kXRows .. kXWave64.  NR rows are loaded per step: the RPL
+// owned rows and NH halo rows.
+template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE>
 struct StreamLane {
+    static constexpr int NH = XMODE == kXWave64 ? 2 : 1;
+    static constexpr int NR = RPL + NH;
     // Buffer resources over the two cell buffers: a 128-bit access whose byte
     // offset is at or beyond the buffer size reads zeros / is dropped by the
     // hardware range check, which is how rows and planes outside the grid
     // (y or z halo beyond the faces, groups past the row end) are handled
     // without branches.
-    __amdgpu_buffer_rsrc_t rin, rout;
+    __amdgpu_buffer_rsrc_t rin, rout;  // based at plane zbase of the two buffers
     const StreamArgs &a;
     uint32_t lane, xg, slot, nslots, lpr, vmask;
-    int y_first, z0, z1;
-    bool yok[RPL + 1];
-    uint32_t roff[RPL + 1];  // byte offset of the row within a z plane (kOutOfRange if none)
+    int y_first, z0, z1, zbase;
+    bool yok[NR];
+    uint32_t roff[NR];  // byte offset of the row within a z plane (kOutOfRange if none)
     uint32_t plane_bytes;
     // statistics
     uint32_t surv = 0, setm = 0, dying = 0, nrows = 0, colmask = 0;  // setm: cells set to M (born + surviving)
@@ -120,15 +146,15 @@ struct StreamLane {
     }
 
     __device__ __forceinline__ uint4 load(int r, int z) const
-    {   // z = -1 wraps to a huge offset and z = Z lands at the buffer end: both read zeros
-        return load_at(r, (uint32_t)z * plane_bytes);
+    {   // z = -1 wraps to a huge offset and z = Z lands at the slab end: both read zeros
+        return load_at(r, (uint32_t)(z - zbase) * plane_bytes);
     }
 
     __device__ __forceinline__ void store(int r, int z, const uint32_t P[4]) const
     {
         typedef unsigned int v4u __attribute__((ext_vector_type(4)));
         const v4u v = {P[0], P[1], P[2], P[3]};
-        __builtin_amdgcn_raw_buffer_store_b128(v, rout, (int)(roff[r] + (uint32_t)z * plane_bytes), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(v, rout, (int)(roff[r] + (uint32_t)(z - zbase) * plane_bytes), 0, 0);
     }
 
     __device__ __forceinline__ void planes_of(const uint4 &w, uint32_t P[4]) const
@@ -144,21 +170,22 @@ struct StreamLane {
     // (ci.w), load plane zi+dz into co.w when `more` (the next step consumes
     // it), finalize plane zi-dz when `fin`.  The z stencil is symmetric, so a
     // descending walk computes the same planes.
-    __device__ __forceinline__ void step(int zi, int dz, bool more, bool fin, Carry<RPL> &ci, Carry<RPL> &co)
+    __device__ __forceinline__ void step(int zi, int dz, bool more, bool fin, Carry<RPL, NR> &ci,
+                                         Carry<RPL, NR> &co)
     {
         const RuleConsts &rc = a.rc;
-        RowSums rs[RPL + 1];
-        uint4 wcur[RPL + 1];
+        RowSums rs[NR];
+        uint4 wcur[NR];
 #pragma unroll
-        for (int r = 0; r <= RPL; r++) wcur[r] = ci.w[r];
+        for (int r = 0; r < NR; r++) wcur[r] = ci.w[r];
         // Unconditional: past the walk's last plane the load gets an offset
         // the range check rejects (no memory access, zeros, never consumed).
         // A conditional load would be waited on at once.
-        const uint32_t zoff = more ? (uint32_t)(zi + dz) * plane_bytes : kNoPlane;
+        const uint32_t zoff = more ? (uint32_t)(zi + dz - zbase) * plane_bytes : kNoPlane;
 #pragma unroll
-        for (int r = 0; r <= RPL; r++) co.w[r] = load_at(r, zoff);
+        for (int r = 0; r < NR; r++) co.w[r] = load_at(r, zoff);
 #pragma unroll
-        for (int r = 0; r <= RPL; r++) {
+        for (int r = 0; r < NR; r++) {
             uint32_t P[4];
             planes_of(wcur[r], P);
             const uint32_t A = live_mask_v3(P, rc.nm, rc.en);
@@ -170,9 +197,15 @@ struct StreamLane {
                 co.A[r] = A;
                 co.NZ[r] = nonzero_mask(P);
             }
-            uint32_t prev = dpp_from_prev_lane(A);
-            uint32_t next = dpp_from_next_lane(A);
-            if constexpr (!FULLROW) {  // several x-rows share a 16-lane DPP row
+            uint32_t prev, next;
+            if constexpr (XMODE >= kXWave32) {
+                prev = dpp_wave_from_prev_lane(A);
+                next = dpp_wave_from_next_lane(A);
+            } else {
+                prev = dpp_from_prev_lane(A);
+                next = dpp_from_next_lane(A);
+            }
+            if constexpr (XMODE == kXRows || XMODE == kXWave32) {  // several x-rows share the shifted lanes
                 prev = xg == 0u ? 0u : prev;
                 next = xg == lpr - 1u ? 0u : next;
             }
@@ -186,7 +219,12 @@ struct StreamLane {
         using K2 = decltype(kind2<FLAGS>(rs[0]));
         K1 k1_up, k1_dn;
         K2 k2_up, k2_dn;
-        {
+        if constexpr (NH == 2) {  // this lane loaded both halo rows: rs[RPL] above, rs[RPL + 1] below
+            k1_up = kind1<FLAGS>(rs[RPL]);
+            k1_dn = kind1<FLAGS>(rs[RPL + 1]);
+            k2_up = kind2<FLAGS>(rs[RPL]);
+            k2_dn = kind2<FLAGS>(rs[RPL + 1]);
+        } else {
             const K1 k1_last = kind1<FLAGS>(rs[RPL - 1]), k1_first = kind1<FLAGS>(rs[0]), k1_h = kind1<FLAGS>(rs[RPL]);
             const K2 k2_last = kind2<FLAGS>(rs[RPL - 1]), k2_first = kind2<FLAGS>(rs[0]), k2_h = kind2<FLAGS>(rs[RPL]);
             const int src_up = ((int)lane - (int)lpr) & 63;  // slot - 1, same xg
@@ -265,7 +303,7 @@ struct StreamLane {
 };
 
 
-template <uint32_t FLAGS, int RPL, bool PLANAR, bool FULLROW>
+template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <= 2 ? 4 : 1))) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
                                                      StreamArgs a, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
@@ -285,33 +323,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
     StatAcc acc;
 
     if (gwave < a.total_waves) {
-        const uint32_t buf_bytes = a.G * a.Y * a.Z * 16u;
-        StreamLane<FLAGS, RPL, PLANAR, FULLROW> L{__builtin_amdgcn_make_buffer_rsrc((void *)in, 0, (int)buf_bytes, kBufferDword3),
-                                         __builtin_amdgcn_make_buffer_rsrc((void *)out, 0, (int)buf_bytes, kBufferDword3),
-                                         a};
+        using Lane = StreamLane<FLAGS, RPL, PLANAR, XMODE>;
+        constexpr int NR = Lane::NR;
+        const uint32_t ytile = gwave % a.ytiles;
+        const uint32_t ztile = gwave / a.ytiles;
+        const int z0 = (int)(ztile * a.tz);
+        const int z1 = min(z0 + (int)a.tz, (int)a.Z);
+        // the wave's slab: planes zbase .. zend-1 (its own planes and the halo
+        // planes inside the grid), addressed through descriptors based there
+        const uint32_t plane_bytes = a.G * a.Y * 16u;
+        const int zbase = max(z0 - 1, 0), zend = min(z1 + 1, (int)a.Z);
+        const size_t slab_at = (size_t)zbase * plane_bytes;
+        const int slab_bytes = (int)((uint32_t)(zend - zbase) * plane_bytes);
+        Lane L{__builtin_amdgcn_make_buffer_rsrc((void *)((const char *)in + slab_at), 0, slab_bytes, kBufferDword3),
+               __builtin_amdgcn_make_buffer_rsrc((void *)((char *)out + slab_at), 0, slab_bytes, kBufferDword3), a};
         L.lane = threadIdx.x & 63u;
         L.lpr = 1u << a.lpr_log2;
         L.xg = L.lane & (L.lpr - 1u);
         L.slot = L.lane >> a.lpr_log2;
         L.nslots = 64u >> a.lpr_log2;
-        const uint32_t ytile = gwave % a.ytiles;
-        const uint32_t ztile = gwave / a.ytiles;
         L.y_first = (int)(ytile * a.rows_per_wave + L.slot * RPL);
-        L.z0 = (int)(ztile * a.tz);
-        L.z1 = min(L.z0 + (int)a.tz, (int)a.Z);
+        L.z0 = z0;
+        L.z1 = z1;
+        L.zbase = zbase;
         const bool gvalid = L.xg < a.G;
         L.vmask = !gvalid ? 0u : (L.xg + 1u < a.G ? ~0u : a.last_mask);
         const int y_halo = L.slot == 0 ? L.y_first - 1 : (L.slot == L.nslots - 1 ? L.y_first + RPL : -1);
-        L.plane_bytes = a.G * a.Y * 16u;
+        L.plane_bytes = plane_bytes;
 #pragma unroll
-        for (int r = 0; r <= RPL; r++) {
-            const int y = r < RPL ? L.y_first + r : y_halo;
+        for (int r = 0; r < NR; r++) {
+            // rows RPL (and RPL + 1 with two halo rows): the halo above, then below
+            const int y = r < RPL ? L.y_first + r
+                                  : (Lane::NH == 2 ? (r == RPL ? L.y_first - 1 : L.y_first + RPL) : y_halo);
             L.yok[r] = gvalid && y >= 0 && y < (int)a.Y;
             L.roff[r] = L.yok[r] ? ((uint32_t)y * a.G + L.xg) * 16u : kOutOfRange;
             if (r < RPL) L.yany[r] = 0u;
         }
 
-        Carry<RPL> c0, c1;
+        Carry<RPL, NR> c0, c1;
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
             for (int k = 0; k < 4; k++) c0.P[r][k] = 0u;
@@ -327,7 +376,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
         const int zs = dz > 0 ? L.z0 : L.z1 - 1;  // first owned plane of the walk
         const int n = L.z1 - L.z0;                 // owned planes
 #pragma unroll
-        for (int r = 0; r <= RPL; r++) {
+        for (int r = 0; r < NR; r++) {
             c0.w[r] = L.load(r, zs - dz);
         }
 
@@ -466,10 +515,15 @@ static uint32_t il_valid_mask(uint32_t ncells)
     return m;
 }
 
+// Largest slab (z planes a wave addresses through one descriptor) in bytes:
+// below kNoPlane, so an offset of kNoPlane past any row reads nothing.
+constexpr uint64_t kMaxSlabBytes = (1ull << 30) - 1u;
+
 bool stream_supported(const GridGeom &g)
-{   // rows of <= 16 groups (one DPP row) and buffers addressable by 32-bit offsets below kOutOfRange / 2
-    return g.X >= 1 && g.X <= 512 && g.pitch == 4u * ((g.X + 31u) / 32u) &&
-           (uint64_t)g.pitch * 4u * g.Y * g.Z < (1ull << 30);
+{   // rows of <= 64 groups (one wave); any number of rows and planes (each wave
+    // addresses its own slab, and a plane is at most 64 MiB: 3 planes fit a slab)
+    return g.X >= 1 && g.X <= 2048 && g.pitch == 4u * ((g.X + 31u) / 32u) &&
+           3ull * g.pitch * 4u * g.Y <= kMaxSlabBytes;
 }
 
 StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t waves)
@@ -479,17 +533,21 @@ StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t wa
     uint32_t lpr = 1;
     while (lpr < G) lpr <<= 1;
     p.lpr = lpr;
-    p.rpl = rpl ? rpl : (lpr >= 8 ? 2u : 1u);
+    // rows of more than 16 groups run two rows per lane only
+    p.rpl = lpr > 16u ? 2u : (rpl ? rpl : (lpr >= 8 ? 2u : 1u));
     p.waves = waves ? waves : 4u;
     const uint32_t rows_per_wave = (64u / lpr) * p.rpl;
     p.ytiles = (g.Y + rows_per_wave - 1) / rows_per_wave;
+    // a wave's slab (tz planes + 2 halo planes) stays below kMaxSlabBytes
+    const uint64_t plane_bytes = (uint64_t)g.pitch * 4u * g.Y;
+    const uint32_t tz_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kMaxSlabBytes / plane_bytes - 2, 1u << 20));
     if (tz) {
-        p.tz = tz;
+        p.tz = std::min(tz, tz_cap);
     } else {
         // aim at >= 16 waves per CU on a 256-CU part (4 per SIMD), with z tiles of >= 8 planes
         uint32_t t = 64;
         while (t > 8 && (uint64_t)p.ytiles * ((g.Z + t - 1) / t) < 4096) t >>= 1;
-        p.tz = std::min(t, g.Z);
+        p.tz = std::min(std::min(t, g.Z), tz_cap);
     }
     p.ztiles = (g.Z + p.tz - 1) / p.tz;
     const uint32_t total = p.ytiles * p.ztiles;
@@ -503,14 +561,18 @@ static void launch_flags(const uint4 *in, uint4 *out, const StreamArgs &a, const
 {
     const dim3 grid(p.blocks), block(64 * p.waves);
     switch (p.rpl) {
-    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done); break;
     case 2:
-        if (p.lpr == 16u)
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, true>), grid, block, 0, st, in, out, a, parts, done);
+        if (p.lpr == 64u)
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave64>), grid, block, 0, st, in, out, a, parts, done);
+        else if (p.lpr == 32u)
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave32>), grid, block, 0, st, in, out, a, parts, done);
+        else if (p.lpr == 16u)
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXFullRow>), grid, block, 0, st, in, out, a, parts, done);
         else
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done);
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done);
         break;
-    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR, false>), grid, block, 0, st, in, out, a, parts, done); break;
+    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done); break;
     }
 }
 
@@ -534,6 +596,9 @@ int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
                   StatPart *partials, hipStream_t st, const uint32_t *done)
 {
     if (p.rpl != 1 && p.rpl != 2 && p.rpl != 4) return fail(CAVE_E_INVALID, "rows per lane must be 1, 2 or 4");
+    if (p.lpr > 16u && p.rpl != 2) return fail(CAVE_E_INVALID, "rows of more than 512 cells run 2 rows per lane");
+    if ((uint64_t)(p.tz + 2u) * g.pitch * 4u * g.Y > kMaxSlabBytes)
+        return fail(CAVE_E_INVALID, "z tile too deep: a wave's slab must stay below 1 GiB");
     if (p.waves < 1 || p.waves > 4) return fail(CAVE_E_INVALID, "waves per block must be 1..4");
     StreamArgs a{};
     a.X = g.X;

@@ -98,7 +98,8 @@ typedef struct CaveChunkConfiguration {
 
 /* Per-tick statistics: the reference counters plus live / changed counts
  * and the bounding box of live (state == M) cells after the tick.
- * bbox_min[a] > bbox_max[a] means no live cell. */
+ * bbox_min[a] > bbox_max[a] means no live cell.  The counters are u32, as the
+ * reference's are: on grids of 2^32 cells or more they wrap modulo 2^32. */
 typedef struct CaveTickStats {
     CaveGridSnapshot snap;
     uint32_t live;    /* born + surviving */
@@ -147,7 +148,7 @@ enum {
 enum {
     CAVE_KERNEL_AUTO = 0,
     CAVE_KERNEL_GATHER = 1,        /* one lane per packed word, per-cell gather            */
-    CAVE_KERNEL_STREAM = 2,        /* bit-sliced z-streaming kernel, bit-planar cell groups */
+    CAVE_KERNEL_STREAM = 2,        /* bit-sliced z-streaming kernel, bit-planar cell groups (X <= 2048) */
     CAVE_KERNEL_STREAM_NIBBLE = 3, /* the same kernel on the reference nibble layout        */
     CAVE_KERNEL_ERD = 4            /* the ERD shape (shape id 1), linear layout              */
 };
@@ -206,8 +207,8 @@ int cave_step_destroy(CaveStep *h);
  *     from cells_a, so the state after the call is in cells_a when `ticks` is
  *     even and in cells_b when it is odd.
  *   kernel: CAVE_KERNEL_AUTO (the streaming kernel on the nibble layout when
- *     ceil(X/8) is a multiple of 4, X <= 512, no W and a buffer below 1 GiB,
- *     else the gather kernel), CAVE_KERNEL_GATHER or CAVE_KERNEL_STREAM_NIBBLE.
+ *     ceil(X/8) is a multiple of 4, X <= 2048 and no W, else the gather
+ *     kernel), CAVE_KERNEL_GATHER or CAVE_KERNEL_STREAM_NIBBLE.
  *   per_tick: nullable; `ticks` CaveTickStats in memory the device can write
  *     (device memory or mapped pinned host memory).
  *   stream: a hipStream_t (NULL: the null stream).  Work is queued on it and

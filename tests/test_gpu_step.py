@@ -60,6 +60,10 @@ RANDOM_CASES = [
     # variant of the streaming kernel, every neighbourhood
     (300, 9, 6, 3, 1), (300, 9, 6, 4, 2), (300, 9, 6, 5, 3), (300, 9, 6, 6, 4), (300, 9, 6, 2, 5),
     (300, 9, 6, 15, 6), (300, 9, 6, 7, 7), (257, 5, 4, 1, 7), (511, 4, 3, 9, 3), (512, 6, 5, 7, 7),
+    # rows of 513..1024 cells take 32 lanes (two x-rows per wave) and rows of
+    # 1025..2048 a whole wave: x-neighbours cross 16-lane rows with wave shifts
+    (513, 9, 4, 3, 7), (600, 9, 5, 4, 3), (777, 10, 3, 2, 5), (1024, 9, 4, 6, 7), (1025, 5, 4, 1, 1),
+    (1100, 5, 6, 7, 7), (1500, 7, 3, 15, 2), (2047, 5, 4, 5, 6), (2048, 5, 3, 3, 7), (2048, 3, 17, 2, 4),
 ]
 
 

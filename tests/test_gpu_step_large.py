@@ -76,3 +76,59 @@ def test_256_multi_tile(kernel, flags):
 def test_ragged_multi_tile(kernel):
     b, s = busy_rule()
     compare_run((200, 130, 70), 4, FEV, b, s, 0.3, 5, kernel, 6)
+
+
+# ---- rows wider than 512 cells, large buffers, more than 2^32 cells ----
+
+@pytest.mark.parametrize("kernel", [cv.KERNEL_STREAM, cv.KERNEL_STREAM_NIBBLE], ids=["stream", "stream_nibble"])
+@pytest.mark.parametrize("dims,M,flags", [
+    ((600, 96, 80), 5, FEV),                   # 32 lanes per x-row, partial last group
+    ((1024, 128, 96), 3, cv.FACES | cv.EDGES),  # 32 lanes per x-row, full
+    ((1100, 70, 40), 7, FEV),                  # a whole wave per x-row, partial
+    ((2048, 64, 48), 2, cv.FACES),             # a whole wave per x-row, full
+], ids=["600", "1024", "1100", "2048"])
+def test_wide_rows(kernel, dims, M, flags):
+    b, s = busy_rule()
+    n = O.lib().orc_neighbor_count(flags)
+    mask = (1 << n) - 1
+    compare_run(dims, M, flags, b & mask, s & mask, 0.25, 21, kernel, 5)
+
+
+def compare_words(dims, M, flags, birth, surv, ticks, seed):
+    """Random 32-bit words as the initial cells: with M = 15 every nibble is a
+    state (0 dead, 1..14 dying, 15 live), and X a multiple of 8 leaves no row
+    padding.  The fast CPU form starts from the same words."""
+    X, Y, Z = dims
+    assert X % 8 == 0
+    print(f"case {X}x{Y}x{Z} M={M} flags={flags} random words, ticks={ticks}", flush=True)
+    words = np.random.default_rng(seed).integers(0, 1 << 32, size=O.cell_words(X, Y, Z), dtype=np.uint64)
+    words = words.astype(np.uint32)
+    fast = O.FastGrid(words, X, Y, Z)
+    with cv.Step(dims, M, flags, birth, surv, initial_cells=words, kernel=cv.KERNEL_STREAM) as h:
+        assert h.kernel == cv.KERNEL_STREAM
+        del words
+        want = fast.run(M, flags, birth, surv, ticks, n_threads=cpu_threads(), stats=True)
+        got, _ = h.run(ticks)
+        for t in range(ticks):
+            assert got[t].as_tuple() == want[t].as_tuple(), f"tick {t}"
+        assert np.array_equal(h.cells(), fast.cells()), f"cells after {ticks} ticks"
+    fast.close()
+
+
+def test_1024_cubed():
+    compare_words((1024, 1024, 1024), 15, FEV, cv.rule_string_to_bitmask("1,2"),
+                  cv.rule_string_to_bitmask("1-3"), 3, 31)
+
+
+def test_buffer_beyond_1gib():
+    # 1024 x 1024 x 2100: 1.10 GB per state buffer
+    compare_words((1024, 1024, 2100), 15, cv.FACES | cv.EDGES, cv.rule_string_to_bitmask("2,3"),
+                  cv.rule_string_to_bitmask("1,2"), 2, 32)
+
+
+def test_more_than_2_32_cells():
+    """2048 x 1024 x 2049 = 4.30e9 cells (2.15 GB per buffer): the per-tick
+    counters wrap modulo 2^32 on both sides, as the reference's u32
+    counters do."""
+    compare_words((2048, 1024, 2049), 15, FEV, cv.rule_string_to_bitmask("1"), cv.rule_string_to_bitmask("1,2"),
+                  2, 33)
