@@ -113,7 +113,8 @@ def run_step_variant(dims, kernel_name: str, steps: int, warmup: int):
     cells = dims[0] * dims[1] * dims[2]
     launch_s = dev_ms / 1e3 / steps
     gbs = cells / launch_s / 1e9
-    summary = {((GRID, GRID, 4 * GRID), "stream"): "step_stream_512x512x2048_FEV_C8.md"}.get((tuple(dims), kernel_name))
+    summary = {((GRID, GRID, 4 * GRID), "stream"): "step_stream_512x512x2048_FEV_C8.md",
+               ((1024, 1024, 1024), "stream"): "step_stream_1024_FEV_C8.md"}.get((tuple(dims), kernel_name))
     return {"grid": list(dims), "kernel": f"step_{kernel_name}", "ticks": steps, "warmup": warmup,
             "traffic": load_traffic(summary) if summary else None,
             "avg_launch_us": launch_s * 1e6, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
@@ -519,9 +520,11 @@ def main():
         "device": {"name": dev, "cus": cus, "hbm_gib": round(mem / 2**30, 1)},
     }
     if not a.no_variants:
-        log("step variants: 512x512x2048 (beyond the Infinity Cache), nibble layout at 512^3")
+        log("step variants: 512x512x2048 (beyond the Infinity Cache), nibble layout at 512^3, large grids")
         out["step_beyond_cache"] = run_step_variant((GRID, GRID, 4 * GRID), "stream", 200, 200)
         out["step_nibble"] = run_step_variant((GRID, GRID, GRID), "stream_nibble", 500, 500)
+        # the reference's large-grid test sizes (src/testing/testRunner.cpp:200-229) and 1024^3
+        out["step_large_grids"] = [run_step_variant((e, e, e), "stream", 50, 50) for e in (950, 1024, 1500, 2023)]
     model, ncpu = cpu_info()
     if not a.no_cpu:
         log("host-CPU baseline on the single-grid workload")

@@ -132,3 +132,53 @@ def test_more_than_2_32_cells():
     counters do."""
     compare_words((2048, 1024, 2049), 15, FEV, cv.rule_string_to_bitmask("1"), cv.rule_string_to_bitmask("1,2"),
                   2, 33)
+
+
+@pytest.mark.parametrize("edge", [950, 1500, 2023])
+def test_reference_large_grid_sizes(edge):
+    """The reference's large-grid workloads (src/testing/testRunner.cpp:200-229,
+    RunSimulationTicks at :21-90): the render path's initial cells (unsalted
+    hash, density 0.5) in a centred spawn box of edge/5, birth {4}, survival
+    {4}, max cell state 5, F neighbourhood, no wrap, 5 ticks.  Activity
+    spreads at most one cell per tick beyond the box, so: the fast CPU form
+    reruns a window of the box grown by ticks + 1 cells (cut from the GPU's
+    own initial cells, whose init kernel the smaller tests pin against the
+    oracle) and must match the GPU's window; every cell outside the window
+    must be 0 before and after; and the whole-grid counters must be the
+    window's plus the outside cells, dead (modulo 2^32 on 2023^3)."""
+    ticks, M, flags = 5, 5, cv.FACES
+    birth, surv = cv.rule_string_to_bitmask("4"), cv.rule_string_to_bitmask("4")
+    E, S = edge, edge // 5
+    lo, hi = (E - S) // 2, (E + S) // 2
+    w0, w1 = max(0, lo - ticks - 1), min(E, hi + ticks + 1)
+    x0, x1 = w0 // 8 * 8, min(-(-w1 // 8) * 8, -(-E // 8) * 8)
+    upr = -(-E // 8)
+    WX, WY, WZ = x1 - x0, w1 - w0, w1 - w0
+    print(f"reference large-grid case {E}^3 spawn {S}: window x {x0}..{x1}, y/z {w0}..{w1}", flush=True)
+
+    def window(words):
+        g = words.reshape(E, E, upr)
+        win = np.ascontiguousarray(g[w0:w1, w0:w1, x0 // 8:x1 // 8]).reshape(-1)
+        g[w0:w1, w0:w1, x0 // 8:x1 // 8] = 0
+        assert not g.any(), "a cell outside the window is not 0"
+        return win
+
+    with cv.Step((E, E, E), M, flags, birth, surv, spawn=(S, S, S), spawn_mode=cv.SPAWN_RENDER_RANDOM,
+                 density=0.5, kernel=cv.KERNEL_STREAM) as h:
+        init = window(h.cells())
+        assert init.any()
+        fast = O.FastGrid(init, WX, WY, WZ)
+        del init
+        want = fast.run(M, flags, birth, surv, ticks, n_threads=cpu_threads(), stats=True)
+        got, _ = h.run(ticks)
+        # cells of the grid outside the window (window columns past X, if any,
+        # are dead on the CPU side and absent on the GPU side: the same offset)
+        outside = E ** 3 - WX * WY * WZ
+        for t in range(ticks):
+            d, dy, b, s, live, ch, bmin, bmax = want[t].as_tuple()
+            exp = ((d + outside) % 2**32, dy, b, s, live, ch,
+                   tuple(v + o for v, o in zip(bmin, (x0, w0, w0))) if live else bmin,
+                   tuple(v + o for v, o in zip(bmax, (x0, w0, w0))) if live else bmax)
+            assert got[t].as_tuple() == exp, f"tick {t}"
+        assert np.array_equal(window(h.cells()), fast.cells()), "cells after 5 ticks"
+    fast.close()
