@@ -149,7 +149,7 @@ inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t thread
     return job_zchunks(X, Y, zp, threads, ipt) * parts;
 }
 
-const void *batch_kernel();
+const void *batch_kernel(bool split);
 const void *serve_kernel();
 int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
 // split_jobs: the batch holds jobs split over several blocks (selects the

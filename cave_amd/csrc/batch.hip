@@ -399,6 +399,22 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch,
     }
 }
 
+// The parts of a split job are consecutive queue entries, claimed in order:
+// every part has been claimed once the claim counter (the work-queue counter,
+// or the served ring's head) reaches `need`, the index one past the job's
+// last part.  Until then a part may be waiting for partners that have not
+// started because every block is busy, which is not a hang: the parts' wait
+// limit only runs from there on.  (Every split job has at most a quarter of
+// the resident blocks as parts, and the entries in flight always form one
+// contiguous range of the queue, so the unclaimed parts are claimed as the
+// blocks ahead of them finish.)
+template <bool SERVE>
+__device__ __forceinline__ bool parts_all_claimed(const BatchArgs &a, uint64_t need)
+{
+    if constexpr (SERVE) return __hip_atomic_load(&a.sd->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need;
+    else return (uint64_t)__hip_atomic_load(a.next_job, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need;
+}
+
 // Rendezvous of the parts of a multi-part job at the end of a phase (the
 // init, then each tick).  Every wave first waits until its device-scope
 // stores (the shared boundary planes) have completed; thread 0 then adds the
@@ -407,11 +423,13 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch,
 // block reads is written and read with device-scope atomics, and each wave's
 // stores complete (vmcnt 0) before the arrival is issued, so a part that has
 // seen every arrival reads the others' planes of this phase.  A wait is
-// bounded: past `wait_limit` the job is aborted for all parts.  Returns false
-// (to every thread) when the job was aborted.
-__device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t phase, const uint32_t part_sum[3],
-                                const int32_t bmin[3], const int32_t bmax[3], uint32_t tot[3], int32_t tmin[3],
-                                int32_t tmax[3], uint32_t *s_flag)
+// bounded: once every part of the job has been claimed (see
+// parts_all_claimed), a wait longer than `wait_limit` aborts the job for all
+// parts.  Returns false (to every thread) when the job was aborted.
+template <bool SERVE>
+__device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
+                                const uint32_t part_sum[3], const int32_t bmin[3], const int32_t bmax[3],
+                                uint32_t tot[3], int32_t tmin[3], int32_t tmax[3], uint32_t *s_flag)
 {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -439,9 +457,11 @@ __device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t ph
         __builtin_amdgcn_s_waitcnt(0);
         __hip_atomic_fetch_add(&js->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t target = j.parts * (phase + 1u);
-        const uint64_t t0 = wall_clock64();
+        uint64_t t0 = wall_clock64();
+        bool started = false;
         uint32_t ok = 1u;
         while (__hip_atomic_load(&js->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (!started && !(started = parts_all_claimed<SERVE>(a, need))) t0 = wall_clock64();
             if (__hip_atomic_load(&js->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                 wall_clock64() - t0 > a.wait_limit) {
                 __hip_atomic_store(&js->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -473,8 +493,10 @@ __device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint32_t ph
 // atomics, then arrives, polls and reads the totals back: two more round
 // trips per tick.)  The slot of phase + 1 (= phase - 2) was last read before
 // every part arrived for phase - 1: part 0 clears it before it arrives.
-__device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint32_t phase, uint32_t born,
-                                     uint32_t surv, uint32_t *born_tot, uint32_t *surv_tot, uint32_t *s_flag)
+template <bool SERVE>
+__device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
+                                     uint32_t born, uint32_t surv, uint32_t *born_tot, uint32_t *surv_tot,
+                                     uint32_t *s_flag)
 {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -487,9 +509,11 @@ __device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint32
         uint64_t *w = &js->tick[phase % 3u];
         const uint64_t add = (uint64_t)surv | ((uint64_t)born << 24) | ((uint64_t)1 << 48);
         uint64_t v = __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
-        const uint64_t t0 = wall_clock64();
+        uint64_t t0 = wall_clock64();
+        bool started = false;
         uint32_t ok = 1u;
         while ((uint32_t)(v >> 48) < j.parts) {
+            if (!started && !(started = parts_all_claimed<SERVE>(a, need))) t0 = wall_clock64();
             if (__hip_atomic_load(&js->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                 wall_clock64() - t0 > a.wait_limit) {
                 __hip_atomic_store(&js->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -531,6 +555,8 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     const JobGeo g = job_geo(j);
     const RuleConsts rc = rule_consts_for(j.flags, j.M, j.born_lut, j.surv_lut);
     const bool multi = j.parts > 1u;
+    // the claim counter value at which every part of this job has been claimed
+    const uint64_t need = sh.claim + (uint64_t)(j.parts - j.part);
     uint32_t *S = j.tier == kTierOwnRegion ? reinterpret_cast<uint32_t *>(j.region) : arena;
     {   // the same in every lane: kept in scalar registers for the global tier's accesses
         const uint64_t sa = reinterpret_cast<uint64_t>(S);
@@ -550,7 +576,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     uint32_t t3[3] = {0u, 0u, 0u};
     int32_t b3[3] = {0, 0, 0}, c3[3] = {0, 0, 0};
     bool ok = true;
-    if (multi) ok = part_rendezvous(a, j, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
+    if (multi) ok = part_rendezvous<SERVE>(a, j, need, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
     // Split jobs whose per-tick totals fit one word take the one-atomic
     // per-tick rendezvous; their dying totals and the last tick's bounding box
@@ -580,12 +606,12 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
                 part_dying_all += tot[0];
                 part_dying_last = tot[0];
             }
-            ok = part_rendezvous_tick(a, j, t + 1u, tot[1], tot[2], &tot[1], &tot[2], &sh.ok);
+            ok = part_rendezvous_tick<SERVE>(a, j, need, t + 1u, tot[1], tot[2], &tot[1], &tot[2], &sh.ok);
             if (!ok) break;
         } else if (multi) {
             // every tick: whether this tick is the last is known only after the totals
             block_bbox(acc, sh.box, bmin, bmax);
-            ok = part_rendezvous(a, j, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
+            ok = part_rendezvous<SERVE>(a, j, need, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
             if (!ok) break;
         }
         if (threadIdx.x == 0) {
@@ -636,7 +662,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         int32_t bmin[3], bmax[3];
         block_bbox(acc, sh.box, bmin, bmax);
         uint32_t fin[3] = {part_dying_all, part_dying_last, 0u};
-        ok = part_rendezvous(a, j, phase, fin, bmin, bmax, fin, bmin, bmax, &sh.ok);
+        ok = part_rendezvous<SERVE>(a, j, need, phase, fin, bmin, bmax, fin, bmin, bmax, &sh.ok);
         phase++;
         if (ok && threadIdx.x == 0) {
             const uint32_t born = last.snap.numberOfCellsBorn, surv = last.snap.numberOfSurvivingCells;
@@ -658,7 +684,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     if constexpr (SERVE) {
         // the final cells of every part reach memory before part 0 reports
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        if (multi && ok) ok = part_rendezvous(a, j, phase, t3, b3, c3, t3, b3, c3, &sh.ok);
+        if (multi && ok) ok = part_rendezvous<SERVE>(a, j, need, phase, t3, b3, c3, t3, b3, c3, &sh.ok);
         else __syncthreads();
     }
     if (threadIdx.x == 0 && j.part == 0u) {
@@ -821,7 +847,7 @@ int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t
     return CAVE_OK;
 }
 
-const void *batch_kernel() { return (const void *)k_batch<false>; }
+const void *batch_kernel(bool split) { return split ? (const void *)k_batch<true> : (const void *)k_batch<false>; }
 
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
                  bool split_jobs)

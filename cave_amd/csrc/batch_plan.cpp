@@ -180,15 +180,23 @@ uint32_t cu_count()
     return (uint32_t)cus;
 }
 
+// Blocks per CU that stay resident, for the form of the batch kernel with
+// the lower occupancy (a plan is sized before it knows whether it splits a
+// job, which picks the form launched).
 uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes)
 {
-    int per = 0;
-    if (lds_bytes > 64u * 1024u)
-        (void)hipFuncSetAttribute(batch_kernel(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(), (int)threads, lds_bytes) != hipSuccess ||
-        per < 1)
-        per = 1;
-    return (uint32_t)per;
+    int best = 1 << 30;
+    for (bool split : {false, true}) {
+        int per = 0;
+        if (lds_bytes > 64u * 1024u)
+            (void)hipFuncSetAttribute(batch_kernel(split), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(split), (int)threads, lds_bytes) !=
+                hipSuccess ||
+            per < 1)
+            per = 1;
+        best = std::min(best, per);
+    }
+    return (uint32_t)best;
 }
 
 uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z)
