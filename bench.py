@@ -264,6 +264,62 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     return out
 
 
+def run_reference_sweep(partitions=(1, 64)):
+    """The reference's own published workload, run as its release benchmark
+    script runs it (benchmarks/scripts/release/bench_grand_release.ps1:30-66):
+    `--mode search --shape cube --grid 49 --neighborhood FE
+    --chunks-per-config P --search-workgroup-size 32 --max-rule-bits 2
+    --min-cs 2 --max-cs-range 3 --ticks 20 --dual-gpu --headless`, one fresh
+    process per P, timed as whole-process wall (start-up, device
+    initialisation and output files included), the per-chunk counts summed
+    from the log with the script's regex.  71,532 rule configs per sweep; the
+    reference's published rates are 339 configs/s (Release, P = 1;
+    bench_grand_results.csv:2) and 792 unique configs/s (Debug, P = 64;
+    bench_grand_results_DEBUG.csv:8), both on 2x Quadro RTX 8000."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    cli = os.path.join(ROOT, "cave_amd", "_lib", "cave")
+    with open(os.path.join(ROOT, "tests", "golden", "search_counts.json")) as f:
+        want = json.load(f)["oracle"]["49"]
+    rx = re.compile(r"viable: (\d+), possible: (\d+), unviable: (\d+).+_c(\d+)_.+_gpu(\d+)_viable")
+    published = {1: 339.0, 64: 792.0}
+    out = {"workload": "cube 49^3, FE, --max-rule-bits 2, --min-cs 2 --max-cs-range 3, --ticks 20, 11^3 filled "
+                       "spawn (CLI defaults), whole-process wall per sweep", "runs": []}
+    for P in partitions:
+        d = tempfile.mkdtemp(prefix="cave_sweep_")
+        try:
+            t0 = time.perf_counter()
+            r = subprocess.run([cli, "--mode", "search", "--shape", "cube", "--grid", "49", "--neighborhood", "FE",
+                                "--chunks-per-config", str(P), "--search-workgroup-size", "32", "--max-rule-bits",
+                                "2", "--min-cs", "2", "--max-cs-range", "3", "--ticks", "20", "--dual-gpu",
+                                "--headless", "--output", d], capture_output=True, text=True, timeout=600)
+            wall = time.perf_counter() - t0
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cave --mode search failed ({r.returncode}): {r.stderr[-2000:]}")
+        tot, seen = [0, 0, 0], set()
+        for line in r.stdout.splitlines():
+            m = rx.search(line)
+            if m and (m.group(4), m.group(5)) not in seen:
+                seen.add((m.group(4), m.group(5)))
+                for k in range(3):
+                    tot[k] += int(m.group(k + 1))
+        n = sum(tot)
+        match = tot == [want["viable"], want["possible"], want["unviable"]]
+        if not match:
+            log(f"SWEEP COUNT MISMATCH at P={P}: {tot} vs the oracle's {want}")
+        out["runs"].append({"chunks_per_config": P, "chunks": len(seen), "rule_configs": n, "wall_s": wall,
+                            "configs_per_s": n / wall, "reference_configs_per_s": published[P],
+                            "vs_reference": n / wall / published[P],
+                            "counts": {"viable": tot[0], "possible": tot[1], "unviable": tot[2]},
+                            "counts_match_oracle": match})
+    return out
+
+
 EDGE_BUCKETS = ["24-31", "32-47", "48-63", "64-95", "96-127", "128-160"]
 FLAG_NAMES = {1: "F", 3: "FE", 7: "FEV"}
 
@@ -464,6 +520,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the 512x512x2048 and nibble-layout step timings")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-reference-sweep", action="store_true")
     ap.add_argument("--plain-configs", type=int, default=64)
     ap.add_argument("--serve-split-us", type=int, default=0,
                     help="split target of the server (0 = its default)")
@@ -539,6 +596,9 @@ def main():
         if not a.no_uneven:
             log("rule search, uneven sizes: 2048 configs")
             out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
+        if not a.no_reference_sweep:
+            log("the reference's published sweep (49^3 FE, K = 2, cs 2-3, 20 ticks) through the CLI")
+            out["reference_sweep"] = run_reference_sweep()
     if not a.no_serve:
         log(f"serving: Poisson {a.serve_rate:.0f} req/s for {a.serve_seconds:.0f} s")
         out["serving"] = run_serving_bench(a.serve_rate, a.serve_seconds, cpu=not a.no_cpu,

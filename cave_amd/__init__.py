@@ -157,6 +157,8 @@ _SIGS = {
     "cave_packed_to_render": (C.c_int, [_u32, C.POINTER(_u32), _u32, _u32, _u32, _vp]),
     "cave_search_create": (C.c_int, [C.POINTER(SearchSpec), C.POINTER(_vp)]),
     "cave_search_step": (C.c_int, [_vp, _u64]),
+    "cave_search_step_group": (C.c_int, [C.POINTER(_vp), _sz, _u64]),
+    "cave_estimate_chunk_vram_bytes": (_u64, [_u32, _u32, _u32, _u32]),
     "cave_search_is_running": (C.c_int, [_vp]),
     "cave_search_progress": (C.c_int, [_vp, C.POINTER(SearchProgress)]),
     "cave_search_drain": (C.c_int, [_vp, C.POINTER(SearchResult), _sz, C.POINTER(_sz)]),
@@ -536,6 +538,17 @@ class Search:
             self.close()
         except Exception:
             pass
+
+
+def estimate_chunk_vram_bytes(X: int, Y: int, Z: int, slot_count: int) -> int:
+    """PermutationMath::EstimateChunkVramBytes (permutationMath.h:144-153)."""
+    return int(lib().cave_estimate_chunk_vram_bytes(X, Y, Z, slot_count))
+
+
+def search_step_group(chunks, max_evals: int = 0) -> None:
+    """Evaluates the pending rules of several Search chunks in shared launches."""
+    arr = (C.c_void_p * max(len(chunks), 1))(*[c._h for c in chunks])
+    check(lib().cave_search_step_group(arr, len(chunks), max_evals))
 
 
 # ---- continuous-batching request server ----

@@ -214,18 +214,40 @@ int cave_search_create(const CaveSearchSpec *spec, CaveSearch **out)
     return CAVE_OK;
 }
 
-int cave_search_step(CaveSearch *h, uint64_t max_evals)
+// PermutationMath::EstimateChunkVramBytes (permutationMath.h:144-153): three
+// packed cell buffers of ceil(V/8) words and ~128 B of small buffers per slot,
+// plus ~16 MiB of pipeline and descriptor objects per chunk.
+uint64_t cave_estimate_chunk_vram_bytes(uint32_t X, uint32_t Y, uint32_t Z, uint32_t slot_count)
 {
-    if (!h) return fail(CAVE_E_INVALID, "null handle");
+    const uint64_t packed = ((uint64_t)X * Y * Z + 7u) / 8u;
+    return (3u * 4u * packed + 128u) * slot_count + (16ull << 20);
+}
+
+int cave_search_step(CaveSearch *h, uint64_t max_evals) { return cave_search_step_group(&h, 1, max_evals); }
+
+int cave_search_step_group(CaveSearch *const *chunks, size_t n_chunks, uint64_t max_evals)
+{
+    if (!chunks && n_chunks) return fail(CAVE_E_INVALID, "null chunk list");
+    for (size_t c = 0; c < n_chunks; c++)
+        if (!chunks[c]) return fail(CAVE_E_INVALID, "null handle");
     const uint64_t kBatch = 1u << 20;
     uint64_t budget = max_evals ? max_evals : ~0ull;
     std::vector<CaveSimSpec> specs;
     std::vector<PendingEval> evals;
+    std::vector<uint32_t> owner;  // chunk of each evaluation
+    size_t cur = 0;               // chunks before `cur` have nothing left to enumerate
     while (budget > 0) {
         specs.clear();
         evals.clear();
+        owner.clear();
         PendingEval e{};
-        while (specs.size() < std::min<uint64_t>(kBatch, budget) && advance(h, e)) {
+        while (specs.size() < std::min<uint64_t>(kBatch, budget) && cur < n_chunks) {
+            CaveSearch *h = chunks[cur];
+            if (!advance(h, e)) {
+                cur++;
+                continue;
+            }
+            owner.push_back((uint32_t)cur);
             const Slot &s = h->slots[e.slot];
             CaveSimSpec sp{};
             sp.dims[0] = h->spec.chunk.gridDimensionX;
@@ -252,6 +274,7 @@ int cave_search_step(CaveSearch *h, uint64_t max_evals)
         int rc = cave_batch_run(specs.data(), specs.size(), res.data(), &opt);
         if (rc) return rc;
         for (size_t i = 0; i < specs.size(); i++) {
+            CaveSearch *h = chunks[owner[i]];
             const CaveSimSpec &sp = specs[i];
             CaveSearchResult r{};
             // bitmask form (searchSystem.cpp:668-685)
@@ -271,8 +294,9 @@ int cave_search_step(CaveSearch *h, uint64_t max_evals)
             if (h->files[r.category].is_open()) h->files[r.category] << jsonl_line(r) << '\n';
             h->ready.push_back(r);
         }
-        for (auto &f : h->files)
-            if (f.is_open()) f.flush();
+        for (size_t c = 0; c < n_chunks; c++)
+            for (auto &f : chunks[c]->files)
+                if (f.is_open()) f.flush();
         budget -= specs.size();
     }
     return CAVE_OK;

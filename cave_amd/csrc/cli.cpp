@@ -102,7 +102,9 @@ const char *kHelp =
     "          --spawn-mode random|filled, --spawn-density d | --density-sweep a:b:s,\n"
     "          --seed-runs R, --output DIR, --legacy-death\n"
     "          --shape cube|ed (W in --neighborhood adds wrapping slots)\n"
-    "          (accepted and ignored: --headless --dual-gpu --search-workgroup-size --gpu-vram-budget-gb)\n"
+    "          --gpu-vram-budget-gb G (default 40): chunks admitted together under the reference's\n"
+    "          VRAM estimate share launches; a chunk above it alone aborts the sweep\n"
+    "          (accepted and ignored: --headless --dual-gpu --search-workgroup-size)\n"
     "  step:   --grid N, --birth \"5,6\", --survival \"4-6\", --max-cs M, --neighborhood FE[W], --frames T,\n"
     "          --spawn S, --spawn-mode random|filled, --spawn-density d, --seed s, --dump FILE\n"
     "          (--mode render --headless: the same loop with the render path's unsalted random spawn;\n"
@@ -245,58 +247,123 @@ int run_search(const Args &a)
     std::filesystem::create_directories(folder);
     const uint32_t sim_flags = a.flag("legacy-death") ? CAVE_SIM_LEGACY_DEATH : 0u;
 
-    size_t n_chunks = grids.size() * densities.size() * (size_t)seed_runs * (size_t)P;
-    log_info("Starting search sweep: " + std::to_string(n_chunks) + " chunk(s), maxCS " + std::to_string(min_cs) + "-" +
-             std::to_string(max_cs) + ", output: " + folder);
-    std::map<RuleKey, RuleStats> agg;
-    uint64_t tot[3] = {0, 0, 0};
-    uint32_t ordinal = 0;
-    const auto t0 = std::chrono::steady_clock::now();
+    // the reference's slot list, which its VRAM estimate counts: every M of
+    // the range x every non-empty subset of the selected F / E / C, each again
+    // with W when W is selected (searchMode.cpp:49-102)
+    uint32_t n_subsets = 0;
+    for (uint32_t subset = 1; subset <= (sel & 7u); subset++)
+        if ((subset & sel) == subset) n_subsets++;
+    const uint32_t slot_count = (uint32_t)(max_cs - min_cs + 1) * n_subsets * ((sel & CAVE_WRAP) ? 2u : 1u);
+    // searchMode.cpp:227-229 (float budget, at least 0.5 GB)
+    const uint64_t budget =
+        (uint64_t)(std::max(0.5f, (float)a.real("gpu-vram-budget-gb", 40.0)) * 1024.0 * 1024.0 * 1024.0);
+
+    struct Chunk {
+        uint32_t grid, density;
+        int seed, part;
+    };
+    std::vector<Chunk> chunks;
     for (uint32_t g : grids)
         for (uint32_t d : densities)
             for (int s = 0; s < seed_runs; s++)
-                for (int p = 0; p < P; p++) {
-                    CaveSearchSpec spec{};
-                    CaveChunkConfiguration &c = spec.chunk;
-                    c.shapeId = shape_id;
-                    c.gridDimensionX = c.gridDimensionY = c.gridDimensionZ = g;
-                    c.spawnAreaDimensionX = c.spawnAreaDimensionY = c.spawnAreaDimensionZ = std::min((uint32_t)spawn, g);
-                    c.spawnRegionShapeId = random ? 0u : 1u;
-                    c.spawnDensityIndex = d;
-                    c.runSeedIndex = (uint32_t)s;
-                    c.partitionIndex = (uint32_t)p;
-                    c.partitionCount = (uint32_t)P;
-                    spec.neighborhood_flags = sel;
-                    spec.max_rule_bits = (uint32_t)K;
-                    spec.min_max_cell_state = min_cs;
-                    spec.max_max_cell_state = max_cs;
-                    spec.max_ticks = (uint32_t)ticks;
-                    spec.sim_flags = sim_flags;
-                    ordinal++;
-                    const std::string prefix = base + "_c" + std::to_string(ordinal) + "_g" + std::to_string(g) + "x" +
-                                               std::to_string(g) + "x" + std::to_string(g) + "_d" + std::to_string(d) +
-                                               "_s" + std::to_string(s) + "_gpu0";
-                    spec.output_folder = folder.c_str();
-                    spec.file_prefix = prefix.c_str();
-                    CaveSearch *h = nullptr;
-                    check(cave_search_create(&spec, &h));
-                    check(cave_search_step(h, 0));
-                    std::vector<CaveSearchResult> res(4096);
-                    size_t n = 0;
-                    do {
-                        check(cave_search_drain(h, res.data(), res.size(), &n));
-                        for (size_t i = 0; i < n; i++) aggregate(agg, res[i]);
-                    } while (n == res.size());
-                    CaveSearchProgress pr{};
-                    check(cave_search_progress(h, &pr));
-                    tot[0] += pr.viable;
-                    tot[1] += pr.possible;
-                    tot[2] += pr.unviable;
-                    char msg[4096];
-                    check(cave_search_write_summary(h, msg, sizeof msg));
-                    log_info(msg);
-                    cave_search_destroy(h);
+                for (int p = 0; p < P; p++) chunks.push_back({g, d, s, p});
+    char head[512];
+    std::snprintf(head, sizeof head, "Starting search sweep: %zu chunk(s), maxCS %d-%d, VRAM budget %.1f GB/GPU, output: %s",
+                  chunks.size(), min_cs, max_cs, budget / 1073741824.0, folder.c_str());
+    log_info(head);
+    std::map<RuleKey, RuleStats> agg;
+    uint64_t tot[3] = {0, 0, 0};
+    const auto t0 = std::chrono::steady_clock::now();
+    // The reference keeps as many chunks in flight as its VRAM estimate
+    // admits (searchMode.cpp:349-404); here the admitted chunks share batched
+    // launches.  A chunk that alone exceeds the budget aborts the rest of the
+    // sweep with the reference's error (searchMode.cpp:358-366).
+    size_t i = 0;
+    while (i < chunks.size()) {
+        size_t j = i;
+        uint64_t used = 0;
+        bool abort_sweep = false;
+        while (j < chunks.size()) {
+            const uint32_t g = chunks[j].grid;
+            const uint64_t est = cave_estimate_chunk_vram_bytes(g, g, g, slot_count);
+            if (est > budget) {
+                if (j == i) {
+                    char m[256];
+                    std::snprintf(m, sizeof m,
+                                  "Chunk requires %.2f GB but VRAM budget is only %.2f GB per GPU. Aborting sweep.",
+                                  est / 1073741824.0, budget / 1073741824.0);
+                    log_error(m);
+                    abort_sweep = true;
                 }
+                break;
+            }
+            if (used + est > budget) break;
+            used += est;
+            j++;
+        }
+        if (abort_sweep) break;
+        std::vector<CaveSearch *> hs;
+        std::vector<std::string> prefixes;
+        prefixes.reserve(j - i);
+        for (size_t k = i; k < j; k++) {
+            const Chunk &ck = chunks[k];
+            CaveSearchSpec spec{};
+            CaveChunkConfiguration &c = spec.chunk;
+            c.shapeId = shape_id;
+            c.gridDimensionX = c.gridDimensionY = c.gridDimensionZ = ck.grid;
+            c.spawnAreaDimensionX = c.spawnAreaDimensionY = c.spawnAreaDimensionZ = std::min((uint32_t)spawn, ck.grid);
+            c.spawnRegionShapeId = random ? 0u : 1u;
+            c.spawnDensityIndex = ck.density;
+            c.runSeedIndex = (uint32_t)ck.seed;
+            c.partitionIndex = (uint32_t)ck.part;
+            c.partitionCount = (uint32_t)P;
+            spec.neighborhood_flags = sel;
+            spec.max_rule_bits = (uint32_t)K;
+            spec.min_max_cell_state = min_cs;
+            spec.max_max_cell_state = max_cs;
+            spec.max_ticks = (uint32_t)ticks;
+            spec.sim_flags = sim_flags;
+            const std::string gs = std::to_string(ck.grid);
+            prefixes.push_back(base + "_c" + std::to_string(k + 1) + "_g" + gs + "x" + gs + "x" + gs + "_d" +
+                               std::to_string(ck.density) + "_s" + std::to_string(ck.seed) + "_gpu0");
+            spec.output_folder = folder.c_str();
+            spec.file_prefix = prefixes.back().c_str();
+            CaveSearch *h = nullptr;
+            const int rc = cave_search_create(&spec, &h);
+            if (rc != CAVE_OK) {
+                for (CaveSearch *x : hs) cave_search_destroy(x);
+                check(rc);
+            }
+            hs.push_back(h);
+        }
+        char m[256];
+        std::snprintf(m, sizeof m, "Running chunks %zu-%zu of %zu together (estimated VRAM %.2f/%.2f GB)", i + 1, j,
+                      chunks.size(), used / 1073741824.0, budget / 1073741824.0);
+        log_info(m);
+        const int rc = cave_search_step_group(hs.data(), hs.size(), 0);
+        if (rc != CAVE_OK) {
+            for (CaveSearch *x : hs) cave_search_destroy(x);
+            check(rc);
+        }
+        for (CaveSearch *h : hs) {
+            std::vector<CaveSearchResult> res(4096);
+            size_t n = 0;
+            do {
+                check(cave_search_drain(h, res.data(), res.size(), &n));
+                for (size_t r = 0; r < n; r++) aggregate(agg, res[r]);
+            } while (n == res.size());
+            CaveSearchProgress pr{};
+            check(cave_search_progress(h, &pr));
+            tot[0] += pr.viable;
+            tot[1] += pr.possible;
+            tot[2] += pr.unviable;
+            char msg[4096];
+            check(cave_search_write_summary(h, msg, sizeof msg));
+            log_info(msg);
+            cave_search_destroy(h);
+        }
+        i = j;
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     log_info("Sweep complete. Viable: " + std::to_string(tot[0]) + ", Possible: " + std::to_string(tot[1]) +
              ", Unviable: " + std::to_string(tot[2]));

@@ -334,6 +334,17 @@ typedef struct CaveSearchProgress {
 int cave_search_create(const CaveSearchSpec *spec, CaveSearch **out);
 /* Evaluates up to max_evals pending rules (0 = all) in batched launches. */
 int cave_search_step(CaveSearch *h, uint64_t max_evals);
+/* Evaluates up to max_evals pending rules (0 = all) of several chunks in shared
+ * batched launches, chunk after chunk in list order; each result goes to its
+ * own chunk's counters, drain queue and JSONL files, as with cave_search_step.
+ * The reference runs the chunks its VRAM budget admits side by side on one
+ * GPU (src/modes/searchMode.cpp:349-404); here they share launches. */
+int cave_search_step_group(CaveSearch *const *chunks, size_t n_chunks, uint64_t max_evals);
+/* PermutationMath::EstimateChunkVramBytes (src/common/permutationMath.h:144-153):
+ * the device bytes the reference's SearchSystem takes for one chunk of
+ * slot_count slots, (3 x 4 B x ceil(X*Y*Z/8) + 128 B) per slot + 16 MiB.  The
+ * sweep admits chunks against --gpu-vram-budget-gb with it. */
+uint64_t cave_estimate_chunk_vram_bytes(uint32_t X, uint32_t Y, uint32_t Z, uint32_t slot_count);
 int cave_search_is_running(const CaveSearch *h);
 int cave_search_progress(const CaveSearch *h, CaveSearchProgress *out);
 /* Moves up to cap finished results out; *n receives the count. */

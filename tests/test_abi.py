@@ -109,7 +109,9 @@ def test_batch_of_nothing_is_ok():
 @pytest.mark.parametrize("kw,needle", [
     (dict(dims=(0, 4, 4)), "grid dimensions"),
     (dict(dims=(65537, 1, 1)), "grid dimensions"),
-    (dict(dims=(4096, 4096, 4096)), "2^32"),
+    # cube grids may exceed 2^32 cells (their u32 counters wrap, as the
+    # reference's do); the ERD kernel indexes cells with 32 bits
+    (dict(dims=(4096, 4096, 4096), shape=cave_amd.SHAPE_ERD, kernel=cave_amd.KERNEL_ERD), "2^32"),
     (dict(spawn_mode=99), "spawn_mode"),
     (dict(spawn_mode=cave_amd.SPAWN_BUFFER), "initial_cells"),
     (dict(kernel=99), "kernel selector"),

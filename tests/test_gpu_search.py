@@ -178,3 +178,59 @@ def test_cli_jsonl_bytes(tmp_path):
     assert agg_lines is not None and sorted(agg_lines) == want
     # fractions occur, so the float form is exercised
     assert any("." in l for l in want)
+
+
+def test_grouped_chunks_equal_chunks_alone():
+    """cave_search_step_group (chunks sharing launches, as the reference runs
+    the chunks its VRAM budget admits side by side, searchMode.cpp:349-404)
+    gives every chunk the results it gets alone."""
+    def chunk(p, P):
+        return cv.Search(13, cv.FACES | cv.EDGES, 2, 1, 3, 12, spawn=5, spawn_mode=cv.SPAWN_RANDOM,
+                         density_index=6, seed_index=2, partition_index=p, partition_count=P)
+
+    P = 7
+    alone = []
+    for p in range(P):
+        c = chunk(p, P)
+        c.run()
+        alone.append((c.progress(), results_key(c.drain())))
+        c.close()
+    group = [chunk(p, P) for p in range(P)]
+    cv.search_step_group(group)
+    for p, c in enumerate(group):
+        assert (c.progress(), results_key(c.drain())) == alone[p], f"partition {p}"
+        assert not c.running
+        c.close()
+
+
+def cli_totals(stdout):
+    rx = re.compile(r"viable: (\d+), possible: (\d+), unviable: (\d+).+_c(\d+)_.+_gpu(\d+)_viable")
+    tot, seen = np.zeros(3, np.int64), set()
+    for line in stdout.splitlines():
+        m = rx.search(line)
+        if m and (m.group(4), m.group(5)) not in seen:
+            seen.add((m.group(4), m.group(5)))
+            tot += [int(m.group(1)), int(m.group(2)), int(m.group(3))]
+    return len(seen), list(tot)
+
+
+def test_cli_vram_budget_groups(tmp_path):
+    """--gpu-vram-budget-gb: at 25^3 a chunk of the benchmark sweep is
+    estimated at 16.9 MB (permutationMath.h:144-153), so the default 40 GB
+    admits all 64 chunks into one group and the 0.5 GB minimum 31 per group
+    (3 groups).  Both give the oracle's counts and the same aggregate."""
+    args = ["--mode", "search", "--grid", "25", "--neighborhood", "FE", "--max-rule-bits", "2", "--min-cs", "2",
+            "--max-cs-range", "3", "--ticks", "20", "--headless"]
+    runs = {}
+    for label, budget in (("wide", "40"), ("narrow", "0.5")):
+        out = subprocess.run([CLI, *args, "--gpu-vram-budget-gb", budget, "--output", str(tmp_path / label)],
+                             capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        groups = [l for l in out.stdout.splitlines() if "Running chunks" in l]
+        assert len(groups) == (1 if label == "wide" else 3), groups
+        n, tot = cli_totals(out.stdout)
+        want = FX["oracle"]["25"]
+        assert n == 64 and tot == [want["viable"], want["possible"], want["unviable"]]
+        agg = tmp_path / label / "cube" / "cs2-3_FE_t20_b2_aggregate.jsonl"
+        runs[label] = sorted(open(agg).read().splitlines())
+    assert runs["wide"] == runs["narrow"]
