@@ -229,6 +229,22 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     res, dev_ms = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT)
     wall = time.perf_counter() - t0
     agg = W.aggregate_by_config(specs, res, seeds)
+    # one aggregate JSONL record per rule config (RuleAggregator,
+    # ruleAggregator.h:69-135): the seeds of a config fold into its record
+    import tempfile
+
+    t1 = time.perf_counter()
+    ra = cv.Aggregate()
+    ra.record_batch(specs, res)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cfg3_aggregate.jsonl")
+        ra.write_jsonl(path)
+        with open(path) as f:
+            records = sum(1 for _ in f)
+    jsonl_ms = (time.perf_counter() - t1) * 1e3
+    ra.close()
+    unique_rules = len({(int(b), int(s), int(m)) for b, s, m in
+                        zip(specs["birth"][::seeds], specs["survival"][::seeds], specs["M"][::seeds])})
     updates = int(res["ticks_run"].astype(np.int64).sum()) * cells
     out = {
         "metric": "rule-configs/sec at 49^3x20 ticks",
@@ -241,6 +257,9 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
         "early_exit": "reference status-1 early death on",
         "classes": {"viable": int(agg["viable"].sum()), "possible": int(agg["possible"].sum()),
                     "unviable": int(agg["unviable"].sum())},
+        "aggregate_jsonl": {"records": records, "unique_rule_configs": unique_rules, "host_ms": jsonl_ms,
+                            "note": "one RuleAggregator record per distinct rule (configs drawing the same "
+                                    "rule fold together, as in the reference); written after the timed launch"},
     }
     log(f"  persistent engine {dev_ms:.2f} ms; plain engine on {plain_configs} configs")
     sub = specs[: plain_configs * seeds]

@@ -159,6 +159,12 @@ _SIGS = {
     "cave_search_step": (C.c_int, [_vp, _u64]),
     "cave_search_step_group": (C.c_int, [C.POINTER(_vp), _sz, _u64]),
     "cave_estimate_chunk_vram_bytes": (_u64, [_u32, _u32, _u32, _u32]),
+    "cave_aggregate_create": (C.c_int, [C.POINTER(_vp)]),
+    "cave_aggregate_record": (C.c_int, [_vp, C.POINTER(SimulationParameters), _u32, _u32, _u32]),
+    "cave_aggregate_record_batch": (C.c_int, [_vp, C.POINTER(SimSpec), C.POINTER(SimResult), _sz]),
+    "cave_aggregate_rule_count": (_sz, [_vp]),
+    "cave_aggregate_write_jsonl": (C.c_int, [_vp, C.c_char_p]),
+    "cave_aggregate_destroy": (C.c_int, [_vp]),
     "cave_search_is_running": (C.c_int, [_vp]),
     "cave_search_progress": (C.c_int, [_vp, C.POINTER(SearchProgress)]),
     "cave_search_drain": (C.c_int, [_vp, C.POINTER(SearchResult), _sz, C.POINTER(_sz)]),
@@ -531,6 +537,44 @@ class Search:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().cave_search_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Aggregate:
+    """RuleAggregator (src/common/ruleAggregator.h:24-135): results grouped by
+    rule, written as one JSONL record per rule."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().cave_aggregate_create(C.byref(h)))
+        self._h = h
+
+    def record(self, params: SimulationParameters, ticks_survived: int, category: int, shape_id: int = 0) -> None:
+        check(lib().cave_aggregate_record(self._h, C.byref(params), ticks_survived, category, shape_id))
+
+    def record_batch(self, specs: np.ndarray, results: np.ndarray) -> None:
+        """SIM_DTYPE specs and their RESULT_DTYPE results."""
+        specs = np.ascontiguousarray(specs, dtype=SIM_DTYPE)
+        results = np.ascontiguousarray(results, dtype=RESULT_DTYPE)
+        assert len(specs) == len(results)
+        check(lib().cave_aggregate_record_batch(self._h, specs.ctypes.data_as(C.POINTER(SimSpec)),
+                                                results.ctypes.data_as(C.POINTER(SimResult)), len(specs)))
+
+    def __len__(self) -> int:
+        return int(lib().cave_aggregate_rule_count(self._h))
+
+    def write_jsonl(self, path: str) -> None:
+        check(lib().cave_aggregate_write_jsonl(self._h, path.encode()))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cave_aggregate_destroy(self._h)
             self._h = None
 
     def __del__(self):

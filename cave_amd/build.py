@@ -36,6 +36,7 @@ LIB_SOURCES = [
     "api_batch.cpp",
     "api_serve.cpp",
     "api_search.cpp",
+    "api_aggregate.cpp",
     "batch_plan.cpp",
     "batch.hip",
     "step_gather.hip",

@@ -110,46 +110,6 @@ const char *kHelp =
     "          (--mode render --headless: the same loop with the render path's unsalted random spawn;\n"
     "           W in --neighborhood wraps every axis)\n";
 
-// ---- RuleAggregator (src/common/ruleAggregator.h:24-135) ----
-struct RuleStats {
-    uint32_t total = 0, viable = 0, possible = 0, unviable = 0;
-    uint32_t max_ticks = 0, min_ticks = UINT32_MAX;
-    uint64_t sum_ticks = 0;
-};
-using RuleKey = std::tuple<uint64_t, uint64_t, uint32_t, uint32_t, uint32_t>;
-
-void aggregate(std::map<RuleKey, RuleStats> &agg, const CaveSearchResult &r)
-{
-    const uint64_t pm = (uint64_t(1) << 60) - 1;
-    const RuleKey k{r.permutation.birthAndMaxCellStateRules & pm, r.permutation.survivalAndNeighborhoodRules & pm,
-                    (uint32_t)((r.permutation.birthAndMaxCellStateRules >> 60) & 0xF) + 1u,
-                    (uint32_t)(r.permutation.survivalAndNeighborhoodRules >> 60),
-                    (uint32_t)(r.permutation.shapeAndGridConfiguration & 0x1F)};
-    RuleStats &s = agg[k];
-    s.total++;
-    if (r.category == 2) s.viable++;
-    else if (r.category == 1) s.possible++;
-    else s.unviable++;
-    s.max_ticks = std::max(s.max_ticks, r.ticksSurvived);
-    s.min_ticks = std::min(s.min_ticks, r.ticksSurvived);
-    s.sum_ticks += r.ticksSurvived;
-}
-
-void write_aggregate(const std::map<RuleKey, RuleStats> &agg, const std::string &path)
-{
-    std::ofstream o(path, std::ios::out | std::ios::trunc);
-    for (const auto &[k, s] : agg) {
-        const float avg = s.total ? (float)s.sum_ticks / (float)s.total : 0.0f;
-        const float rate = s.total ? (float)s.viable / (float)s.total : 0.0f;
-        o << "{\"birth\":" << std::get<0>(k) << ",\"survival\":" << std::get<1>(k)
-          << ",\"maxCellState\":" << std::get<2>(k) << ",\"neighborhoodFlags\":" << std::get<3>(k)
-          << ",\"shapeId\":" << std::get<4>(k) << ",\"totalRuns\":" << s.total << ",\"viableRuns\":" << s.viable
-          << ",\"possibleRuns\":" << s.possible << ",\"unviableRuns\":" << s.unviable
-          << ",\"viabilityRate\":" << rate << ",\"minTicksSurvived\":" << (s.min_ticks == UINT32_MAX ? 0 : s.min_ticks)
-          << ",\"maxTicksSurvived\":" << s.max_ticks << ",\"avgTicksSurvived\":" << avg << "}\n";
-    }
-}
-
 uint32_t parse_flags(const std::string &nb, bool allow_wrap)
 {
     uint32_t f = 0;
@@ -271,7 +231,9 @@ int run_search(const Args &a)
     std::snprintf(head, sizeof head, "Starting search sweep: %zu chunk(s), maxCS %d-%d, VRAM budget %.1f GB/GPU, output: %s",
                   chunks.size(), min_cs, max_cs, budget / 1073741824.0, folder.c_str());
     log_info(head);
-    std::map<RuleKey, RuleStats> agg;
+    // RuleAggregator (src/common/ruleAggregator.h:24-135), in the library
+    CaveAggregate *agg = nullptr;
+    check(cave_aggregate_create(&agg));
     uint64_t tot[3] = {0, 0, 0};
     const auto t0 = std::chrono::steady_clock::now();
     // The reference keeps as many chunks in flight as its VRAM estimate
@@ -350,7 +312,9 @@ int run_search(const Args &a)
             size_t n = 0;
             do {
                 check(cave_search_drain(h, res.data(), res.size(), &n));
-                for (size_t r = 0; r < n; r++) aggregate(agg, res[r]);
+                for (size_t r = 0; r < n; r++)
+                    check(cave_aggregate_record(agg, &res[r].permutation, res[r].ticksSurvived, res[r].category,
+                                                (uint32_t)(res[r].permutation.shapeAndGridConfiguration & 0x1Fu)));
             } while (n == res.size());
             CaveSearchProgress pr{};
             check(cave_search_progress(h, &pr));
@@ -368,8 +332,9 @@ int run_search(const Args &a)
     log_info("Sweep complete. Viable: " + std::to_string(tot[0]) + ", Possible: " + std::to_string(tot[1]) +
              ", Unviable: " + std::to_string(tot[2]));
     const std::string agg_path = folder + base + "_aggregate.jsonl";
-    write_aggregate(agg, agg_path);
-    log_info("Aggregated " + std::to_string(agg.size()) + " unique rules to " + agg_path);
+    check(cave_aggregate_write_jsonl(agg, agg_path.c_str()));
+    log_info("Aggregated " + std::to_string(cave_aggregate_rule_count(agg)) + " unique rules to " + agg_path);
+    cave_aggregate_destroy(agg);
     char line[256];
     std::snprintf(line, sizeof line, "Evaluated %llu rule configs in %.3f s (%.0f configs/s)",
                   (unsigned long long)(tot[0] + tot[1] + tot[2]), secs, (tot[0] + tot[1] + tot[2]) / std::max(secs, 1e-9));

@@ -353,6 +353,23 @@ int cave_search_drain(CaveSearch *h, CaveSearchResult *out, size_t cap, size_t *
 int cave_search_write_summary(CaveSearch *h, char *msg, size_t cap);
 int cave_search_destroy(CaveSearch *h);
 
+/* ---- rule aggregate (RuleAggregator, src/common/ruleAggregator.h:24-135) ----
+ * Results grouped by rule (birth and survival bitmasks, max cell state M,
+ * neighbourhood flags, shape id): runs per category and the min / max / sum
+ * of ticks survived; written as one JSONL line per rule with the reference's
+ * keys, in key order (the reference's map is unordered). */
+typedef struct CaveAggregate CaveAggregate;
+int cave_aggregate_create(CaveAggregate **out);
+/* One result, as RuleAggregator::Record: category 2 viable, 1 possible, 0 unviable. */
+int cave_aggregate_record(CaveAggregate *a, const CaveSimulationParameters *params, uint32_t ticks_survived,
+                          uint32_t category, uint32_t shape_id);
+/* n batch results: each spec's rule (shape id 1 for CAVE_SIM_SHAPE_ERD specs),
+ * ticks survived and classification.  Seeds of one rule config fold into one record. */
+int cave_aggregate_record_batch(CaveAggregate *a, const CaveSimSpec *specs, const CaveSimResult *results, size_t n);
+size_t cave_aggregate_rule_count(const CaveAggregate *a);
+int cave_aggregate_write_jsonl(const CaveAggregate *a, const char *path);
+int cave_aggregate_destroy(CaveAggregate *a);
+
 /* ================================================================== */
 /* Continuous-batching request server                                  */
 /* ================================================================== */
