@@ -339,6 +339,39 @@ def run_reference_sweep(partitions=(1, 64)):
     return out
 
 
+def run_erd_search_bench(n_configs=4096, seeds=32, plain_configs=64):
+    """The uniform rule search on the ERD cell shape (49^3, F + E = 12 face +
+    4 edge neighbours, <= 2 + 2 rule bits over counts 1..16, C in 2..5, 32
+    seeds, 20 ticks) on the persistent engine's bit-sliced ERD tick, beside
+    the per-config engine (the reference-form ERD kernel, one step and one
+    manager launch per simulation per tick) on the first configs."""
+    import numpy as np
+
+    import cave_amd as cv
+    from cave_amd import workloads as W
+
+    specs = W.search_uniform_erd(n_configs, seeds)
+    cells = 49 ** 3
+    cv.batch_run_np(specs[: seeds * 8])  # warm-up
+    res, dev_ms = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT)
+    updates = int(res["ticks_run"].astype(np.int64).sum()) * cells
+    sub = specs[: plain_configs * seeds]
+    pres, pms = cv.batch_run_np(sub, engine=cv.ENGINE_PLAIN)
+    same = bool(np.array_equal(pres["ticks_run"], res["ticks_run"][: len(sub)]) and
+                np.array_equal(pres["final_alive"], res["final_alive"][: len(sub)]))
+    if not same:
+        log("ERD persistent / per-config engines disagree")
+    return {"metric": "rule-configs/sec at 49^3x20 ticks, ERD cell shape",
+            "value": n_configs / (dev_ms / 1e3), "unit": "configs/s", "configs": n_configs,
+            "seeds_per_config": seeds, "device_ms": dev_ms, "cell_updates_per_s": updates / (dev_ms / 1e3),
+            "hbm_equivalent_frac": updates / (dev_ms / 1e3) / (HBM_PEAK_GBS * 1e9),
+            "plain": {"value": plain_configs / (pms / 1e3), "unit": "configs/s", "device_ms": pms,
+                      "sample": f"first {plain_configs} configs x {seeds} seeds, the reference-form ERD kernel: "
+                                "one step + one manager launch per sim per tick",
+                      "results_equal": same},
+            "speedup_vs_plain": (n_configs / (dev_ms / 1e3)) / (plain_configs / (pms / 1e3))}
+
+
 EDGE_BUCKETS = ["24-31", "32-47", "48-63", "64-95", "96-127", "128-160"]
 FLAG_NAMES = {1: "F", 3: "FE", 7: "FEV"}
 
@@ -540,6 +573,7 @@ def main():
                     help="skip the 512x512x2048 and nibble-layout step timings")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-reference-sweep", action="store_true")
+    ap.add_argument("--no-erd", action="store_true")
     ap.add_argument("--plain-configs", type=int, default=64)
     ap.add_argument("--serve-split-us", type=int, default=0,
                     help="split target of the server (0 = its default)")
@@ -615,6 +649,9 @@ def main():
         if not a.no_uneven:
             log("rule search, uneven sizes: 2048 configs")
             out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
+        if not a.no_erd:
+            log("rule search on the ERD cell shape: 4096 configs x 32 seeds at 49^3")
+            out["search_erd"] = run_erd_search_bench(plain_configs=a.plain_configs)
         if not a.no_reference_sweep:
             log("the reference's published sweep (49^3 FE, K = 2, cs 2-3, 20 ticks) through the CLI")
             out["reference_sweep"] = run_reference_sweep()

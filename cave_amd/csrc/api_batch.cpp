@@ -247,17 +247,33 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
 
 std::string tuning_report();
 
-// Cube specs on the persistent engine, ERD specs on the per-config one;
-// results, per-tick rows and timing scattered back to spec order.
-int batch_mixed(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
+// Engine groups of a batch: cube specs and non-wrapping ERD specs each run as
+// one persistent launch (the kernel is instantiated per cell shape); wrapping
+// ERD specs (W) take the per-config engine.
+enum { kGroupCube = 0, kGroupErd = 1, kGroupPlain = 2 };
+
+int spec_group(const CaveSimSpec &s)
 {
-    std::vector<size_t> idx[2];
-    for (size_t i = 0; i < n; i++) idx[(specs[i].flags & CAVE_SIM_SHAPE_ERD) ? 1 : 0].push_back(i);
+    if (!(s.flags & CAVE_SIM_SHAPE_ERD)) return kGroupCube;
+    return (s.neighborhood_flags & CAVE_WRAP) ? kGroupPlain : kGroupErd;
+}
+
+// Runs each group of the batch on its engine; results, per-tick rows and
+// timing scattered back to spec order.  The device time is the groups' sum.
+int batch_grouped(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
+{
+    std::vector<size_t> idx[3];
+    for (size_t i = 0; i < n; i++) idx[spec_group(specs[i])].push_back(i);
+    for (int g = 0; g < 3; g++)
+        if (idx[g].size() == n)  // one group: no scatter
+            return g == kGroupPlain ? batch_plain(specs, n, results, opt, st)
+                                    : batch_persistent(specs, n, results, opt, st);
     const uint32_t stride = opt ? opt->per_tick_stride : 0u;
     const bool want_pt = opt && opt->per_tick && stride;
     double total_ms = 0;
-    for (int part = 0; part < 2; part++) {
-        const std::vector<size_t> &ix = idx[part];
+    for (int g = 0; g < 3; g++) {
+        const std::vector<size_t> &ix = idx[g];
+        if (ix.empty()) continue;
         std::vector<CaveSimSpec> sub(ix.size());
         for (size_t k = 0; k < ix.size(); k++) sub[k] = specs[ix[k]];
         std::vector<CaveSimResult> res(ix.size());
@@ -267,11 +283,11 @@ int batch_mixed(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
         if (opt) o = *opt;
         o.per_tick = want_pt ? pt.data() : nullptr;
         o.timing = tm.empty() ? nullptr : tm.data();
-        int rc = part == 0 ? batch_persistent(sub.data(), sub.size(), res.data(), &o, st)
-                           : batch_plain(sub.data(), sub.size(), res.data(), &o, st);
+        int rc = g == kGroupPlain ? batch_plain(sub.data(), sub.size(), res.data(), &o, st)
+                                  : batch_persistent(sub.data(), sub.size(), res.data(), &o, st);
         if (rc) return rc;
         total_ms += o.device_ms;
-        if (part == 0 && opt) opt->block_threads = o.block_threads;
+        if (g != kGroupPlain && opt) opt->block_threads = o.block_threads;
         for (size_t k = 0; k < ix.size(); k++) {
             results[ix[k]] = res[k];
             if (want_pt) std::copy(pt.begin() + k * stride, pt.begin() + (k + 1) * stride, opt->per_tick + ix[k] * stride);
@@ -313,17 +329,8 @@ extern "C" int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimR
     if (engine > CAVE_ENGINE_PLAIN) return fail(CAVE_E_INVALID, "bad engine");
     hipStream_t st;
     CAVE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    size_t n_erd = 0;
-    for (size_t i = 0; i < n_specs; i++) n_erd += (specs[i].flags & CAVE_SIM_SHAPE_ERD) ? 1 : 0;
-    int rc;
-    if (engine == CAVE_ENGINE_PLAIN || n_erd == n_specs) {
-        rc = batch_plain(specs, n_specs, results, opt, st);
-    } else if (n_erd == 0) {
-        rc = batch_persistent(specs, n_specs, results, opt, st);
-    } else {
-        // the persistent kernel runs cube cells; ERD specs take the per-config engine
-        rc = batch_mixed(specs, n_specs, results, opt, st);
-    }
+    const int rc = engine == CAVE_ENGINE_PLAIN ? batch_plain(specs, n_specs, results, opt, st)
+                                               : batch_grouped(specs, n_specs, results, opt, st);
     (void)hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     return rc;

@@ -262,8 +262,8 @@ void plan_request(CaveServer *s, Pending &p)
     const bool in_lds = words * 4u <= s->lds_bytes;
     j.tier = in_lds ? kTierLds : kTierOwnRegion;
     const uint32_t bucket = edge_bucket(j.X, j.Y, j.Z);
-    const double r1 = bucket_rate(bucket, j.flags, kThreads, 1, nullptr, false, nullptr);
-    const double r2 = bucket_rate(bucket, j.flags, kThreads, 2, nullptr, false, nullptr);
+    const double r1 = bucket_rate(bucket, tune_kind(j), kThreads, 1, nullptr, false, nullptr);
+    const double r2 = bucket_rate(bucket, tune_kind(j), kThreads, 2, nullptr, false, nullptr);
     const uint32_t ipt = r2 > r1 * 1.03 ? 2u : 1u;
     const double dev_rate = std::max(r1, r2);
     const double block_rate = dev_rate > 0 ? dev_rate / s->resident : kFallbackBlockRate;
@@ -496,6 +496,8 @@ int cave_serve_submit(CaveServer *s, const CaveSimSpec *spec, uint64_t request_i
     if (!s || !spec) return fail(CAVE_E_INVALID, "null argument");
     int rc = validate_sim(*spec);
     if (rc) return rc;
+    if (spec->flags & CAVE_SIM_SHAPE_ERD)  // the resident kernel is the cube shape's
+        return fail(CAVE_E_UNSUPPORTED, "the server runs cube-shaped requests; batch ERD grids with cave_batch_run");
     Pending p{};
     p.id = request_id;
     p.spec = *spec;

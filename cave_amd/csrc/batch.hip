@@ -260,7 +260,162 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
     }
 }
 
+// ---------------------------------------------------------------------------
+// The elongated rhombic dodecahedron (ERD) cell shape
+// (searchSimulationERD.slang:81-97, elongatedRhombicDodecahedron.slang:6-27):
+// 12 face neighbours whose offsets depend on the parity of the cell's row y,
+// 4 edge neighbours, no corners.  Grouped by neighbour row (y', z'), a cell
+// (x, y, z) sees:
+//   row (y, z):         x-1, x+1                    (F)
+//   rows (y+-1, z):     the pair P                  (F)
+//   rows (y, z+-1):     x (F); x-1, x+1 (E)
+//   rows (y+-1, z-1):   the pair P, when y is even  (F)
+//   rows (y+-1, z+1):   the pair P, when y is odd   (F)
+// where P = {x-1, x} for even y and {x, x+1} for odd y.  Per plane zi an item
+// of row y therefore forms
+//   T  = F*A + E*(L+R)          (row y: its part in the cells of planes zi+-1)
+//   Y  = P(y-1) + P(y+1)        (F)
+//   C  = F*((L+R) + Y)          (its part in the cells of plane zi)
+//   Bk = T + [y even] Y         (its part in the cells of plane zi+1)
+//   Fw = T + [y odd] Y          (its part in the cells of plane zi-1)
+// and N(z) = Bk(z-1) + C(z) + Fw(z+1) <= 16: the cube's z-walk with two
+// carried sums, the parity a per-lane mask.  Cells, guard bands and the state
+// update are the cube's.
+struct ECarry {
+    Bits<4> Q;   // Bk of the plane before the last input plane + C of the last
+    Bits<3> Bk;  // Bk of the last input plane
+    uint32_t A;  // own live word of the last input plane
+};
+
 template <uint32_t FLAGS, bool COH, bool GLOBAL>
+__device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
+                                             const RuleConsts &rc, int w, int y, int z0, int z1, StatAcc &acc)
+{
+    constexpr bool F = (FLAGS & 1u) != 0u, E = (FLAGS & 2u) != 0u;
+    const uint32_t odd = (y & 1) ? ~0u : 0u;
+    const uint32_t vmask = (uint32_t)w + 1u < g.W ? ~0u : g.last_mask;
+    uint32_t colmask = 0u;
+    int zmin = 0x7fffffff, zmax = -1;
+    // the running word offsets of run_item
+    const uint32_t wy = (uint32_t)w + g.W * (uint32_t)y;
+    const uint32_t pstride = g.W * g.Y;
+    const uint32_t arow = g.W + 1u;
+    const uint32_t wya = (uint32_t)w + arow * (uint32_t)y;
+    uint32_t aoff = acur + wya + g.aslab * (uint32_t)z0;
+    uint32_t poff = 4u * (wy + pstride * (uint32_t)(z0 - 2));
+    uint32_t noff = anext + wya + arow + g.aslab * (uint32_t)(z0 - 1);
+
+    auto step = [&](int zi, const ECarry &ci, ECarry &co) {
+        const bool shr = COH && (zi == (int)g.zlo - 1 || zi == (int)g.zhi);
+        uint32_t av[3], pv[3], nv[3];
+        if (shr) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * arow);
+                pv[d] = ld_live<COH>(q, true);
+                av[d] = ld_live<COH>(q + 1, true);
+                nv[d] = ld_live<COH>(q + 2, true);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const auto *q = S.at(aoff - 1u + (uint32_t)d * arow);
+                pv[d] = q[0];
+                av[d] = q[1];
+                nv[d] = q[2];
+            }
+        }
+        uint32_t L[3], R[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            L[d] = shift_left_neighbor_nat(av[d], pv[d]);
+            R[d] = shift_right_neighbor_nat(av[d], nv[d]);
+        }
+        const uint32_t a = av[1];
+        auto T = [&]() {
+            if constexpr (F && E) return Bits<2>{{xor3(a, L[1], R[1]), maj3(a, L[1], R[1])}};
+            else if constexpr (E) return Bits<2>{{L[1] ^ R[1], L[1] & R[1]}};
+            else if constexpr (F) return Bits<1>{{a}};
+            else return Bits<0>{};
+        }();
+        Bits<3> Bk, Fw;
+        if constexpr (F) {
+            Bits<2> P[2];
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int d = 2 * k;  // rows y-1 and y+1
+                // odd ? R : L -- the pair's other cell
+                const uint32_t sel = __builtin_amdgcn_bitop3_b32(R[d], L[d], odd, (unsigned char)0xE4);
+                P[k] = Bits<2>{{av[d] ^ sel, av[d] & sel}};
+            }
+            const Bits<3> Y = add2<3>(P[0], P[1]);
+            const Bits<2> LRy{{L[1] ^ R[1], L[1] & R[1]}};
+            const Bits<3> C = add2<3>(LRy, Y);
+            Bits<3> Ye, Yo;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                Ye.b[k] = and_not(Y.b[k], odd);
+                Yo.b[k] = Y.b[k] & odd;
+            }
+            Bk = add2<3>(T, Ye);
+            Fw = add2<3>(T, Yo);
+            co.Q = add2<4>(ci.Bk, C);
+        } else {
+            Bk = add2<3>(T, Bits<0>{});
+            Fw = Bk;
+            co.Q = add2<4>(ci.Bk, Bits<0>{});
+        }
+        co.Bk = Bk;
+        co.A = a;
+        const int z = zi - 1;
+        if (z >= z0) {
+            const Bits<5> n = add2<5>(ci.Q, Fw);
+            uint32_t tb, ts;
+            apply_rule_consts<1>(&n, rc, &tb, &ts);
+            const uint4 pq = S.ld4(poff);
+            uint32_t pl[4] = {pq.x, pq.y, pq.z, pq.w};
+            const uint32_t NZ = nonzero_mask(pl);
+            const uint32_t Al = ci.A;
+            const uint32_t set_m = vmask & ((Al & ts) | (~NZ & tb));
+            const uint32_t surv = set_m & Al;
+            acc.surv += __popc(surv);
+            acc.born += __popc(set_m);  // born + surviving: the tick's caller takes surv out
+            acc.dying += __popc(and_not(NZ, surv));
+            update_planes(pl, set_m, NZ, rc.mk);
+            S.st4(poff, make_uint4(pl[0], pl[1], pl[2], pl[3]));
+            st_live<COH>(S.at(noff), set_m, COH && (z == (int)g.zlo || z + 1 == (int)g.zhi));
+            colmask |= set_m;
+            if (set_m) {
+                zmin = min(zmin, z);
+                zmax = max(zmax, z);
+            }
+        }
+        aoff += g.aslab;
+        poff += 4u * pstride;
+        noff += g.aslab;
+    };
+
+    ECarry c0, c1;
+    for (int k = 0; k < 4; k++) c0.Q.b[k] = 0u;
+    for (int k = 0; k < 3; k++) c0.Bk.b[k] = 0u;
+    c0.A = 0u;
+    int zi = z0 - 1;
+    for (; zi + 1 <= z1; zi += 2) {
+        step(zi, c0, c1);
+        step(zi + 1, c1, c0);
+    }
+    if (zi <= z1) step(zi, c0, c1);
+    if (colmask) {
+        acc.bmin[0] = min(acc.bmin[0], 32 * w + (int)__builtin_ctz(colmask));
+        acc.bmax[0] = max(acc.bmax[0], 32 * w + 31 - (int)__builtin_clz(colmask));
+        acc.bmin[1] = min(acc.bmin[1], y);
+        acc.bmax[1] = max(acc.bmax[1], y);
+        acc.bmin[2] = min(acc.bmin[2], zmin);
+        acc.bmax[2] = max(acc.bmax[2], zmax);
+    }
+}
+
+template <uint32_t FLAGS, bool COH, bool GLOBAL, bool ERD>
 __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                            const RuleConsts &rc, StatAcc &acc)
 {
@@ -271,23 +426,36 @@ __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acu
         const uint32_t zc = g.zc0 + rest / g.Y;
         const int z0 = (int)(zc * g.tzc);
         const int z1 = min(z0 + (int)g.tzc, (int)g.Z);
-        run_item<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
+        if constexpr (ERD) run_item_erd<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
+        else run_item<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
     }
 }
 
-template <bool COH, bool GLOBAL>
+// flags: F|E|C of a cube job, F|E of an ERD job.  The shape is a template
+// parameter (a kernel instantiation per shape): with both shapes' tick code in
+// one kernel this compiler fails ("illegal VGPR to SGPR copy").
+template <bool COH, bool GLOBAL, bool ERD>
 __device__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                          const RuleConsts &rc, StatAcc &acc)
 {
+    if constexpr (ERD) {
+        switch (flags & 3u) {
+        case 1: tick_items<1, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
+        case 2: tick_items<2, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
+        case 3: tick_items<3, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
+        default: tick_items<0, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
+        }
+        return;
+    }
     switch (flags & 7u) {
-    case 1: tick_items<1, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 2: tick_items<2, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 3: tick_items<3, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 4: tick_items<4, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 5: tick_items<5, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 6: tick_items<6, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    case 7: tick_items<7, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
-    default: tick_items<0, COH, GLOBAL>(g, S, acur, anext, rc, acc); break;
+    case 1: tick_items<1, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 2: tick_items<2, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 3: tick_items<3, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 4: tick_items<4, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 5: tick_items<5, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 6: tick_items<6, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 7: tick_items<7, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    default: tick_items<0, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
     }
 }
 
@@ -548,7 +716,8 @@ struct BlockShared {
 // and completion flag go to host-coherent memory, after every store of the
 // simulation is visible to the host.
 // FAST_SPLIT: split jobs may take the one-word per-tick rendezvous.
-template <bool SERVE, bool FAST_SPLIT>
+// ERD: the jobs are of the ERD shape (make_job keeps their F|E flags).
+template <bool SERVE, bool FAST_SPLIT, bool ERD>
 __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
 {
     const uint64_t t_start = wall_clock64();
@@ -593,9 +762,9 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
         acc = StatAcc{};
-        if (j.tier == kTierLds) run_tick<false>(j.flags, g, SL, acur, anext, rc, acc);
-        else if (multi) run_tick<true>(j.flags, g, SG, acur, anext, rc, acc);
-        else run_tick<false>(j.flags, g, SG, acur, anext, rc, acc);
+        if (j.tier == kTierLds) run_tick<false, false, ERD>(j.flags, g, SL, acur, anext, rc, acc);
+        else if (multi) run_tick<true, true, ERD>(j.flags, g, SG, acur, anext, rc, acc);
+        else run_tick<false, true, ERD>(j.flags, g, SG, acur, anext, rc, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
         uint32_t tot[3];
@@ -707,8 +876,9 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     __syncthreads();
 }
 
-// SPLIT: the batch holds split jobs (FAST_SPLIT entry code).
-template <bool SPLIT>
+// SPLIT: the batch holds split jobs (FAST_SPLIT entry code).  ERD: its jobs
+// are of the ERD shape (a batch runs one shape).
+template <bool SPLIT, bool ERD>
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];  // 16-byte state accesses
@@ -722,7 +892,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];
-        run_entry<false, SPLIT>(a, j, lds, arena, sh);
+        run_entry<false, SPLIT, ERD>(a, j, lds, arena, sh);
     }
 }
 
@@ -832,7 +1002,7 @@ __global__ void __launch_bounds__(1024) k_serve(BatchArgs a)
             reinterpret_cast<uint32_t *>(&j)[k] =
                 __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t *>(&sh.job)[k]);
         __syncthreads();
-        run_entry<true, true>(a, j, lds, nullptr, sh);
+        run_entry<true, true, false>(a, j, lds, nullptr, sh);
     }
 }
 
@@ -847,15 +1017,19 @@ int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t
     return CAVE_OK;
 }
 
-const void *batch_kernel(bool split) { return split ? (const void *)k_batch<true> : (const void *)k_batch<false>; }
+const void *batch_kernel(bool split, bool erd)
+{
+    if (erd) return split ? (const void *)k_batch<true, true> : (const void *)k_batch<false, true>;
+    return split ? (const void *)k_batch<true, false> : (const void *)k_batch<false, false>;
+}
 
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
-                 bool split_jobs)
+                 bool split_jobs, bool erd)
 {
-    const void *k = split_jobs ? (const void *)k_batch<true> : (const void *)k_batch<false>;
+    const void *k = batch_kernel(split_jobs, erd);
     if (lds_bytes > 64u * 1024u) CAVE_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-    if (split_jobs) hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(threads), lds_bytes, st, a);
-    else hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(threads), lds_bytes, st, a);
+    void *args[] = {(void *)&a};
+    CAVE_HIP(hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds_bytes, st));
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

@@ -94,7 +94,9 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
 {
     const uint32_t copies = cu_count() * std::max(1u, 1024u / key.threads);
     const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
-    const uint32_t N = neighbor_count(key.flags);
+    const bool erd = (key.flags & kTuneErd) != 0u;
+    // ERD: 12 face and 4 edge neighbours (searchSimulationERD.slang:81-97)
+    const uint32_t N = erd ? ((key.flags & 1u) ? 12u : 0u) + ((key.flags & 2u) ? 4u : 0u) : neighbor_count(key.flags);
     const uint64_t all = N ? (N >= 64 ? ~0ull : ((1ull << N) - 1)) : 1ull;
     const double target = 2e9;
     const uint32_t ticks = (uint32_t)std::clamp<double>(target / ((double)copies * (double)cells), 2.0, 64.0);
@@ -107,14 +109,14 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
             s.spawn[a] = std::max(1u, dims[a] / 2u);
         }
         s.max_cell_state = 7;
-        s.neighborhood_flags = key.flags;
+        s.neighborhood_flags = key.flags & 7u;
         s.ticks = ticks;
         s.birth = all;
         s.survival = all;
         s.spawn_mode = CAVE_SPAWN_RANDOM;
         s.density = 0.25;
         s.seed = i;
-        s.flags = CAVE_SIM_NO_EARLY_EXIT;
+        s.flags = CAVE_SIM_NO_EARLY_EXIT | (erd ? CAVE_SIM_SHAPE_ERD : 0u);
     }
     BatchKnobs k;
     k.threads = key.threads;
@@ -155,7 +157,7 @@ uint32_t lds_cap(uint32_t threads)
 double bucket_rate(uint32_t bucket, uint32_t flags, uint32_t threads, uint32_t ipt, const uint32_t dims[3],
                    bool measure, hipStream_t st)
 {
-    const TuneKey key{bucket, flags & 7u, threads, ipt};
+    const TuneKey key{bucket, flags & (kTuneErd | 7u), threads, ipt};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_rates.find(key);
@@ -180,24 +182,25 @@ uint32_t cu_count()
     return (uint32_t)cus;
 }
 
-// Blocks per CU that stay resident, for the form of the batch kernel with
-// the lower occupancy (a plan is sized before it knows whether it splits a
-// job, which picks the form launched).
-uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes)
+// Blocks per CU that stay resident, for the form of the batch kernel (of
+// the plan's cell shape) with the lower occupancy: a plan is sized before it
+// knows whether it splits a job, which picks the form launched.
+uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, bool erd)
 {
     int best = 1 << 30;
     for (bool split : {false, true}) {
         int per = 0;
-        if (lds_bytes > 64u * 1024u)
-            (void)hipFuncSetAttribute(batch_kernel(split), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(split), (int)threads, lds_bytes) !=
-                hipSuccess ||
+        const void *k = batch_kernel(split, erd);
+        if (lds_bytes > 64u * 1024u) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, (int)threads, lds_bytes) != hipSuccess ||
             per < 1)
             per = 1;
         best = std::min(best, per);
     }
     return (uint32_t)best;
 }
+
+uint32_t tune_kind(const DevJob &j) { return (j.sim_flags & CAVE_SIM_SHAPE_ERD) ? (kTuneErd | (j.flags & 3u)) : j.flags & 7u; }
 
 uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z)
 {
@@ -216,7 +219,9 @@ DevJob make_job(const CaveSimSpec &s, uint32_t slot)
     j.X = s.dims[0];
     j.Y = s.dims[1];
     j.Z = s.dims[2];
-    j.flags = s.neighborhood_flags & 7u;
+    // ERD cells have no corner neighbours and the persistent kernel runs ERD
+    // grids without wrap: only F|E matter there
+    j.flags = s.neighborhood_flags & ((s.flags & CAVE_SIM_SHAPE_ERD) ? 3u : 7u);
     j.M = s.max_cell_state;
     const DevRule r = make_rule(s.max_cell_state, s.neighborhood_flags, s.birth, s.survival);
     j.born_lut = r.born_lut;
@@ -232,6 +237,10 @@ DevJob make_job(const CaveSimSpec &s, uint32_t slot)
 int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPlan &plan)
 {
     plan = BatchPlan{};
+    plan.erd = n > 0 && (specs[0].flags & CAVE_SIM_SHAPE_ERD);
+    for (size_t i = 0; i < n; i++)
+        if (((specs[i].flags & CAVE_SIM_SHAPE_ERD) != 0) != plan.erd)
+            return fail(CAVE_E_INVALID, "a persistent launch runs one cell shape");
     plan.jobs.resize(n);
     plan.ipt.resize(n);
     std::vector<uint64_t> words(n);
@@ -244,7 +253,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         words[i] = job_words(j.X, j.Y, j.Z);
         cost[i] = (double)j.X * j.Y * j.Z * j.ticks;
         key_bucket[i] = edge_bucket(j.X, j.Y, j.Z);
-        BucketWork &b = buckets[{key_bucket[i], j.flags}];
+        BucketWork &b = buckets[{key_bucket[i], tune_kind(j)}];
         if (b.work == 0)
             for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
         b.work += cost[i];
@@ -342,7 +351,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         }
     }
     plan.lds_bytes = (uint32_t)(lds_words * 4u);
-    const uint32_t per_cu = resident_per_cu(threads, plan.lds_bytes);
+    const uint32_t per_cu = resident_per_cu(threads, plan.lds_bytes, plan.erd);
     const uint32_t resident = cu_count() * per_cu;
 
     // ---- single-block time estimates, splitting, order ----
@@ -352,7 +361,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     for (size_t i = 0; i < n; i++) {
         const DevJob &j = plan.jobs[i];
         double r = 0;
-        ipt_of[i] = best_ipt(key_bucket[i], j.flags, threads, &r);
+        ipt_of[i] = best_ipt(key_bucket[i], tune_kind(j), threads, &r);
         // a bucket's device-wide rate is shared by `resident` blocks
         t[i] = r > 0 ? cost[i] * resident / r : cost[i];
         total += t[i];
@@ -449,7 +458,7 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
     CAVE_HIP(hipEventCreate(&e0));
     CAVE_HIP(hipEventCreate(&e1));
     CAVE_HIP(hipEventRecord(e0, st));
-    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st, plan.nsync > 0);
+    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st, plan.nsync > 0, plan.erd);
     if (rc) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

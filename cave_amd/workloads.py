@@ -55,6 +55,23 @@ def search_uniform(n_configs: int = 4096, seeds: int = 32, ticks: int = 20, seed
     return a
 
 
+def search_uniform_erd(n_configs: int = 4096, seeds: int = 32, ticks: int = 20, seed: int = 6) -> np.ndarray:
+    """The uniform rule search on the ERD cell shape: n_configs configs, each
+    a 49^3 grid with F and E (12 face + 4 edge neighbours), <= 2 birth and
+    <= 2 survival bits from counts 1..16 (uniform over the 136 masks of that
+    set), C uniform in 2..5, `seeds` seeds per config (central 12^3
+    Bernoulli(0.3)).  Config-major order."""
+    a = search_uniform(n_configs, seeds, ticks, seed)
+    rng = cv.Rng(seed + 1000)
+    vb = cv.enumerate_valid_bitmasks(16, 2)
+    for c in range(n_configs):
+        b, s = vb[rng.next() % len(vb)], vb[rng.next() % len(vb)]
+        a["birth"][c * seeds:(c + 1) * seeds] = b
+        a["survival"][c * seeds:(c + 1) * seeds] = s
+    a["sim_flags"] = cv.SIM_SHAPE_ERD
+    return a
+
+
 def search_uneven(n_configs: int = 2048, seed: int = 4) -> np.ndarray:
     """Batched rule search, uneven sizes: edge log-uniform in [24, 160],
     F : FE : FEV = 1 : 2 : 1, C uniform in 2..16, bits p = 0.1, central cube of

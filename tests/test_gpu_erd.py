@@ -116,3 +116,67 @@ def test_cli_shape_ed(tmp_path):
     assert any(f.endswith("_viable.jsonl") for f in files)
     lines = [json.loads(l) for f in files if f.endswith(".jsonl") and "aggregate" not in f for l in open(folder / f)]
     assert len(lines) == sum(N * N for N in (12, 4, 16))  # K = 1: N masks per side
+
+
+def erd_sims(cases, seed, sim_flags=cv.SIM_FINAL_HASH):
+    rng = np.random.default_rng(seed)
+    sims = []
+    for dims, M, flags, ticks in cases:
+        N = O.lib().orc_erd_neighbor_count(flags)
+        birth = sum(1 << b for b in range(N) if rng.random() < 0.25) or 1
+        surv = sum(1 << b for b in range(N) if rng.random() < 0.3) or 1
+        sims.append(cv.Sim(dims=dims, M=M, flags=flags, ticks=ticks, birth=birth, survival=surv,
+                           spawn=tuple(max(1, d // 2) for d in dims), spawn_mode=cv.SPAWN_RANDOM, density=0.3,
+                           seed=int(rng.integers(0, 1 << 20)), sim_flags=sim_flags | cv.SIM_SHAPE_ERD))
+    return sims
+
+
+# (dims, M, flags, ticks): LDS-resident grids, then grids in global memory
+# (above the LDS share of a 1024-thread block), odd and even row counts
+ERD_BATCH = [((9, 7, 6), 1, 1, 12), ((24, 13, 10), 3, 2, 15), ((40, 21, 17), 5, 3, 20), ((33, 32, 31), 2, 3, 18),
+             ((100, 81, 60), 4, 3, 16), ((70, 64, 50), 7, 1, 14), ((96, 45, 66), 2, 2, 12)]
+
+
+@pytest.mark.parametrize("max_parts", [1, 2, 5], ids=["whole", "2parts", "5parts"])
+def test_erd_persistent_engine_matches_oracle(max_parts):
+    """ERD grids on the persistent batch kernel (its bit-sliced ERD tick), in
+    LDS and in global memory, whole and split over blocks: every per-tick
+    row, the grid info and the final hash against the oracle's
+    orc_run_rule_erd (searchSimulationERD.slang:59-217)."""
+    sims = erd_sims(ERD_BATCH, 11)
+    print(f"erd persistent max_parts={max_parts}", flush=True)
+    res, rows, _ = cv.batch_run(sims, engine=cv.ENGINE_PERSISTENT, per_tick=True, max_parts=max_parts)
+    for i, (s, r) in enumerate(zip(sims, res)):
+        X, Y, Z = s.dims
+        init = O.init_cells_erd(X, Y, Z, s.M, s.spawn, s.spawn_mode, s.density, s.seed)
+        t, info, orows, final = O.run_rule_erd(X, Y, Z, s.M, s.flags, s.birth, s.survival, s.ticks, init)
+        assert r.ticks_run == t, f"sim {i}"
+        for f, _ in O.GridInfo._fields_:
+            assert getattr(r.info, f) == getattr(info, f), (i, f)
+        assert [x.as_tuple() for x in rows[i]] == [x.as_tuple() for x in orows], f"sim {i}"
+        assert r.final_hash == O.erd_hash(final, X, Y, Z), f"sim {i}"
+
+
+def test_erd_persistent_equals_per_config_engine():
+    """Larger ERD grids (up to 150^3, 60 ticks, no early exit) on the
+    persistent kernel and on the per-config engine (the reference-form ERD
+    kernel): same results and final cells."""
+    rng = np.random.default_rng(5)
+    cases = [((int(e), int(e), int(e)), int(rng.integers(1, 16)), int((1, 2, 3)[i % 3]), 60)
+             for i, e in enumerate((48, 64, 90, 121, 150, 77))]
+    sims = erd_sims(cases, 6, sim_flags=cv.SIM_FINAL_HASH | cv.SIM_NO_EARLY_EXIT)
+    a, _, _ = cv.batch_run(sims, engine=cv.ENGINE_PERSISTENT)
+    b, _, _ = cv.batch_run(sims, engine=cv.ENGINE_PLAIN)
+    for i in range(len(sims)):
+        assert a[i].ticks_run == b[i].ticks_run == 60
+        assert bytes(a[i].info) == bytes(b[i].info), f"sim {i}"
+        assert a[i].last.as_tuple() == b[i].last.as_tuple(), f"sim {i}"
+        assert a[i].final_hash == b[i].final_hash, f"sim {i}"
+
+
+def test_server_refuses_erd_requests():
+    s = erd_sims([((8, 8, 8), 2, 3, 4)], 1)[0]
+    with cv.Server() as srv:
+        with pytest.raises(cv.CaveError) as e:
+            srv.submit(s, 1)
+    assert e.value.code == cv.CAVE_E_UNSUPPORTED
