@@ -247,27 +247,17 @@ int batch_plain(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Cave
 
 std::string tuning_report();
 
-// Engine groups of a batch: cube specs and non-wrapping ERD specs each run as
-// one persistent launch (the kernel is instantiated per cell shape); wrapping
-// ERD specs (W) take the per-config engine.
-enum { kGroupCube = 0, kGroupErd = 1, kGroupPlain = 2 };
-
-int spec_group(const CaveSimSpec &s)
-{
-    if (!(s.flags & CAVE_SIM_SHAPE_ERD)) return kGroupCube;
-    return (s.neighborhood_flags & CAVE_WRAP) ? kGroupPlain : kGroupErd;
-}
-
+// A batch runs as one persistent launch per cell shape among its specs
+// (kBatchShape*: cube, ERD, wrapping ERD -- the kernel is instantiated per
+// shape), unless the caller asked for the per-config engine.
 // Runs each group of the batch on its engine; results, per-tick rows and
 // timing scattered back to spec order.  The device time is the groups' sum.
 int batch_grouped(const CaveSimSpec *specs, size_t n, CaveSimResult *results, CaveBatchOptions *opt, hipStream_t st)
 {
     std::vector<size_t> idx[3];
-    for (size_t i = 0; i < n; i++) idx[spec_group(specs[i])].push_back(i);
+    for (size_t i = 0; i < n; i++) idx[spec_shape(specs[i])].push_back(i);
     for (int g = 0; g < 3; g++)
-        if (idx[g].size() == n)  // one group: no scatter
-            return g == kGroupPlain ? batch_plain(specs, n, results, opt, st)
-                                    : batch_persistent(specs, n, results, opt, st);
+        if (idx[g].size() == n) return batch_persistent(specs, n, results, opt, st);  // one group: no scatter
     const uint32_t stride = opt ? opt->per_tick_stride : 0u;
     const bool want_pt = opt && opt->per_tick && stride;
     double total_ms = 0;
@@ -283,11 +273,10 @@ int batch_grouped(const CaveSimSpec *specs, size_t n, CaveSimResult *results, Ca
         if (opt) o = *opt;
         o.per_tick = want_pt ? pt.data() : nullptr;
         o.timing = tm.empty() ? nullptr : tm.data();
-        int rc = g == kGroupPlain ? batch_plain(sub.data(), sub.size(), res.data(), &o, st)
-                                  : batch_persistent(sub.data(), sub.size(), res.data(), &o, st);
+        int rc = batch_persistent(sub.data(), sub.size(), res.data(), &o, st);
         if (rc) return rc;
         total_ms += o.device_ms;
-        if (g != kGroupPlain && opt) opt->block_threads = o.block_threads;
+        if (opt) opt->block_threads = o.block_threads;
         for (size_t k = 0; k < ix.size(); k++) {
             results[ix[k]] = res[k];
             if (want_pt) std::copy(pt.begin() + k * stride, pt.begin() + (k + 1) * stride, opt->per_tick + ix[k] * stride);

@@ -262,8 +262,8 @@ void plan_request(CaveServer *s, Pending &p)
     const bool in_lds = words * 4u <= s->lds_bytes;
     j.tier = in_lds ? kTierLds : kTierOwnRegion;
     const uint32_t bucket = edge_bucket(j.X, j.Y, j.Z);
-    const double r1 = bucket_rate(bucket, tune_kind(j), kThreads, 1, nullptr, false, nullptr);
-    const double r2 = bucket_rate(bucket, tune_kind(j), kThreads, 2, nullptr, false, nullptr);
+    const double r1 = bucket_rate(bucket, tune_kind(j, kBatchShapeCube), kThreads, 1, nullptr, false, nullptr);
+    const double r2 = bucket_rate(bucket, tune_kind(j, kBatchShapeCube), kThreads, 2, nullptr, false, nullptr);
     const uint32_t ipt = r2 > r1 * 1.03 ? 2u : 1u;
     const double dev_rate = std::max(r1, r2);
     const double block_rate = dev_rate > 0 ? dev_rate / s->resident : kFallbackBlockRate;

@@ -149,13 +149,15 @@ inline uint32_t part_zchunks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t thread
     return job_zchunks(X, Y, zp, threads, ipt) * parts;
 }
 
-const void *batch_kernel(bool split, bool erd);
+// Cell shape of a persistent launch (every job of a launch has the same):
+// the cube (W ignored), the ERD shape, the ERD shape with every axis wrapping.
+enum : uint32_t { kBatchShapeCube = 0, kBatchShapeErd = 1, kBatchShapeErdWrap = 2 };
+const void *batch_kernel(bool split, uint32_t shape);
 const void *serve_kernel();
 int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st);
 // split_jobs: the batch holds jobs split over several blocks (selects the
-// kernel form with the one-word per-tick rendezvous).  erd: its jobs are of
-// the ERD shape (every job of a launch has the same shape).
+// kernel form with the one-word per-tick rendezvous).
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
-                 bool split_jobs, bool erd);
+                 bool split_jobs, uint32_t shape);
 
 }  // namespace cave

@@ -281,18 +281,36 @@ __device__ __forceinline__ void run_item(const JobGeo &g, const Words<GLOBAL> &S
 // and N(z) = Bk(z-1) + C(z) + Fw(z+1) <= 16: the cube's z-walk with two
 // carried sums, the parity a per-lane mask.  Cells, guard bands and the state
 // update are the cube's.
+// WRAP (the W flag): every axis wraps, each offset its own term (on an axis
+// shorter than 3 cells the same cell counts once per offset that reaches it).
+// The loads are the same, at wrapped addresses: rows y+-1 and planes z+-1
+// past a face are taken from the opposite face instead of the zero guard
+// band, the first word of a row takes its left neighbour from the row's last
+// word (cell X-1 shifted up to bit 31), and the last word takes the right
+// neighbour of cell X-1 from bit 0 of the row's first word.
 struct ECarry {
     Bits<4> Q;   // Bk of the plane before the last input plane + C of the last
     Bits<3> Bk;  // Bk of the last input plane
     uint32_t A;  // own live word of the last input plane
 };
 
-template <uint32_t FLAGS, bool COH, bool GLOBAL>
+template <uint32_t FLAGS, bool COH, bool GLOBAL, bool WRAP>
 __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                                              const RuleConsts &rc, int w, int y, int z0, int z1, StatAcc &acc)
 {
     constexpr bool F = (FLAGS & 1u) != 0u, E = (FLAGS & 2u) != 0u;
     const uint32_t odd = (y & 1) ? ~0u : 0u;
+    // WRAP: word offsets of the x-neighbour words and of rows y-1 / y+1
+    // from the row's own word, and the valid cells of a row's last word
+    const bool first = w == 0, last = w == (int)g.W - 1;
+    const uint32_t rlast = g.X - 32u * (g.W - 1u);  // 1..32
+    uint32_t dpv = 0xFFFFFFFFu, dnv = 1u, ro[3] = {0u, g.W + 1u, 2u * (g.W + 1u)};
+    if constexpr (WRAP) {
+        if (first) dpv = g.W - 1u;
+        if (last) dnv = 1u - g.W;
+        if (y == 0) ro[0] = g.Y * (g.W + 1u);
+        if (y == (int)g.Y - 1) ro[2] = 2u * (g.W + 1u) - g.Y * (g.W + 1u);
+    }
     const uint32_t vmask = (uint32_t)w + 1u < g.W ? ~0u : g.last_mask;
     uint32_t colmask = 0u;
     int zmin = 0x7fffffff, zmax = -1;
@@ -308,7 +326,17 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
     auto step = [&](int zi, const ECarry &ci, ECarry &co) {
         const bool shr = COH && (zi == (int)g.zlo - 1 || zi == (int)g.zhi);
         uint32_t av[3], pv[3], nv[3];
-        if (shr) {
+        if constexpr (WRAP) {
+            // planes -1 and Z are planes Z-1 and 0
+            const uint32_t zc = zi == -1 ? g.Z * g.aslab : (zi == (int)g.Z ? 0u - g.Z * g.aslab : 0u);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const uint32_t own = aoff + zc + ro[d];
+                pv[d] = ld_live<COH>(S.at(own + dpv), shr);
+                av[d] = ld_live<COH>(S.at(own), shr);
+                nv[d] = ld_live<COH>(S.at(own + dnv), shr);
+            }
+        } else if (shr) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const auto *q = S.at(aoff - 1u + (uint32_t)d * arow);
@@ -328,8 +356,13 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
         uint32_t L[3], R[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            L[d] = shift_left_neighbor_nat(av[d], pv[d]);
-            R[d] = shift_right_neighbor_nat(av[d], nv[d]);
+            if constexpr (WRAP) {
+                L[d] = shift_left_neighbor_nat(av[d], first ? pv[d] << (32u - rlast) : pv[d]);
+                R[d] = last ? (av[d] >> 1) | ((nv[d] & 1u) << (rlast - 1u)) : shift_right_neighbor_nat(av[d], nv[d]);
+            } else {
+                L[d] = shift_left_neighbor_nat(av[d], pv[d]);
+                R[d] = shift_right_neighbor_nat(av[d], nv[d]);
+            }
         }
         const uint32_t a = av[1];
         auto T = [&]() {
@@ -415,7 +448,7 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
     }
 }
 
-template <uint32_t FLAGS, bool COH, bool GLOBAL, bool ERD>
+template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE>
 __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                            const RuleConsts &rc, StatAcc &acc)
 {
@@ -426,7 +459,8 @@ __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acu
         const uint32_t zc = g.zc0 + rest / g.Y;
         const int z0 = (int)(zc * g.tzc);
         const int z1 = min(z0 + (int)g.tzc, (int)g.Z);
-        if constexpr (ERD) run_item_erd<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
+        if constexpr (SHAPE != kBatchShapeCube)
+            run_item_erd<FLAGS, COH, GLOBAL, SHAPE == kBatchShapeErdWrap>(g, S, acur, anext, rc, w, y, z0, z1, acc);
         else run_item<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
     }
 }
@@ -434,28 +468,28 @@ __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acu
 // flags: F|E|C of a cube job, F|E of an ERD job.  The shape is a template
 // parameter (a kernel instantiation per shape): with both shapes' tick code in
 // one kernel this compiler fails ("illegal VGPR to SGPR copy").
-template <bool COH, bool GLOBAL, bool ERD>
+template <bool COH, bool GLOBAL, uint32_t SHAPE>
 __device__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                          const RuleConsts &rc, StatAcc &acc)
 {
-    if constexpr (ERD) {
+    if constexpr (SHAPE != kBatchShapeCube) {
         switch (flags & 3u) {
-        case 1: tick_items<1, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
-        case 2: tick_items<2, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
-        case 3: tick_items<3, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
-        default: tick_items<0, COH, GLOBAL, true>(g, S, acur, anext, rc, acc); break;
+        case 1: tick_items<1, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
+        case 2: tick_items<2, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
+        case 3: tick_items<3, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
+        default: tick_items<0, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
         }
         return;
     }
     switch (flags & 7u) {
-    case 1: tick_items<1, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 2: tick_items<2, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 3: tick_items<3, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 4: tick_items<4, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 5: tick_items<5, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 6: tick_items<6, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    case 7: tick_items<7, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
-    default: tick_items<0, COH, GLOBAL, false>(g, S, acur, anext, rc, acc); break;
+    case 1: tick_items<1, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 2: tick_items<2, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 3: tick_items<3, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 4: tick_items<4, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 5: tick_items<5, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 6: tick_items<6, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 7: tick_items<7, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    default: tick_items<0, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
     }
 }
 
@@ -716,8 +750,8 @@ struct BlockShared {
 // and completion flag go to host-coherent memory, after every store of the
 // simulation is visible to the host.
 // FAST_SPLIT: split jobs may take the one-word per-tick rendezvous.
-// ERD: the jobs are of the ERD shape (make_job keeps their F|E flags).
-template <bool SERVE, bool FAST_SPLIT, bool ERD>
+// SHAPE: the jobs' cell shape (kBatchShape*; make_job keeps F|E of ERD jobs).
+template <bool SERVE, bool FAST_SPLIT, uint32_t SHAPE>
 __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
 {
     const uint64_t t_start = wall_clock64();
@@ -762,9 +796,9 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
         acc = StatAcc{};
-        if (j.tier == kTierLds) run_tick<false, false, ERD>(j.flags, g, SL, acur, anext, rc, acc);
-        else if (multi) run_tick<true, true, ERD>(j.flags, g, SG, acur, anext, rc, acc);
-        else run_tick<false, true, ERD>(j.flags, g, SG, acur, anext, rc, acc);
+        if (j.tier == kTierLds) run_tick<false, false, SHAPE>(j.flags, g, SL, acur, anext, rc, acc);
+        else if (multi) run_tick<true, true, SHAPE>(j.flags, g, SG, acur, anext, rc, acc);
+        else run_tick<false, true, SHAPE>(j.flags, g, SG, acur, anext, rc, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
         uint32_t tot[3];
@@ -876,9 +910,9 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     __syncthreads();
 }
 
-// SPLIT: the batch holds split jobs (FAST_SPLIT entry code).  ERD: its jobs
-// are of the ERD shape (a batch runs one shape).
-template <bool SPLIT, bool ERD>
+// SPLIT: the batch holds split jobs (FAST_SPLIT entry code).  SHAPE: its
+// jobs' cell shape (a launch runs one shape).
+template <bool SPLIT, uint32_t SHAPE>
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];  // 16-byte state accesses
@@ -892,7 +926,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];
-        run_entry<false, SPLIT, ERD>(a, j, lds, arena, sh);
+        run_entry<false, SPLIT, SHAPE>(a, j, lds, arena, sh);
     }
 }
 
@@ -1002,7 +1036,7 @@ __global__ void __launch_bounds__(1024) k_serve(BatchArgs a)
             reinterpret_cast<uint32_t *>(&j)[k] =
                 __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t *>(&sh.job)[k]);
         __syncthreads();
-        run_entry<true, true, false>(a, j, lds, nullptr, sh);
+        run_entry<true, true, kBatchShapeCube>(a, j, lds, nullptr, sh);
     }
 }
 
@@ -1017,16 +1051,20 @@ int launch_serve(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t
     return CAVE_OK;
 }
 
-const void *batch_kernel(bool split, bool erd)
+const void *batch_kernel(bool split, uint32_t shape)
 {
-    if (erd) return split ? (const void *)k_batch<true, true> : (const void *)k_batch<false, true>;
-    return split ? (const void *)k_batch<true, false> : (const void *)k_batch<false, false>;
+    switch (shape) {
+    case kBatchShapeErd: return split ? (const void *)k_batch<true, kBatchShapeErd> : (const void *)k_batch<false, kBatchShapeErd>;
+    case kBatchShapeErdWrap:
+        return split ? (const void *)k_batch<true, kBatchShapeErdWrap> : (const void *)k_batch<false, kBatchShapeErdWrap>;
+    default: return split ? (const void *)k_batch<true, kBatchShapeCube> : (const void *)k_batch<false, kBatchShapeCube>;
+    }
 }
 
 int launch_batch(const BatchArgs &a, uint32_t blocks, uint32_t threads, uint32_t lds_bytes, hipStream_t st,
-                 bool split_jobs, bool erd)
+                 bool split_jobs, uint32_t shape)
 {
-    const void *k = batch_kernel(split_jobs, erd);
+    const void *k = batch_kernel(split_jobs, shape);
     if (lds_bytes > 64u * 1024u) CAVE_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
     void *args[] = {(void *)&a};
     CAVE_HIP(hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds_bytes, st));

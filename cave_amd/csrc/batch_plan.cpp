@@ -96,7 +96,7 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
     const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
     const bool erd = (key.flags & kTuneErd) != 0u;
     // ERD: 12 face and 4 edge neighbours (searchSimulationERD.slang:81-97)
-    const uint32_t N = erd ? ((key.flags & 1u) ? 12u : 0u) + ((key.flags & 2u) ? 4u : 0u) : neighbor_count(key.flags);
+    const uint32_t N = erd ? ((key.flags & 1u) ? 12u : 0u) + ((key.flags & 2u) ? 4u : 0u) : neighbor_count(key.flags & 7u);
     const uint64_t all = N ? (N >= 64 ? ~0ull : ((1ull << N) - 1)) : 1ull;
     const double target = 2e9;
     const uint32_t ticks = (uint32_t)std::clamp<double>(target / ((double)copies * (double)cells), 2.0, 64.0);
@@ -109,7 +109,8 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
             s.spawn[a] = std::max(1u, dims[a] / 2u);
         }
         s.max_cell_state = 7;
-        s.neighborhood_flags = key.flags & 7u;
+        // ERD keys: bit 2 (corners, which ERD cells do not have) marks wrap
+        s.neighborhood_flags = erd ? (key.flags & 3u) | ((key.flags & 4u) ? CAVE_WRAP : 0u) : key.flags & 7u;
         s.ticks = ticks;
         s.birth = all;
         s.survival = all;
@@ -185,12 +186,12 @@ uint32_t cu_count()
 // Blocks per CU that stay resident, for the form of the batch kernel (of
 // the plan's cell shape) with the lower occupancy: a plan is sized before it
 // knows whether it splits a job, which picks the form launched.
-uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, bool erd)
+uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, uint32_t shape)
 {
     int best = 1 << 30;
     for (bool split : {false, true}) {
         int per = 0;
-        const void *k = batch_kernel(split, erd);
+        const void *k = batch_kernel(split, shape);
         if (lds_bytes > 64u * 1024u) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, (int)threads, lds_bytes) != hipSuccess ||
             per < 1)
@@ -200,7 +201,17 @@ uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, bool erd)
     return (uint32_t)best;
 }
 
-uint32_t tune_kind(const DevJob &j) { return (j.sim_flags & CAVE_SIM_SHAPE_ERD) ? (kTuneErd | (j.flags & 3u)) : j.flags & 7u; }
+uint32_t spec_shape(const CaveSimSpec &s)
+{
+    if (!(s.flags & CAVE_SIM_SHAPE_ERD)) return kBatchShapeCube;
+    return (s.neighborhood_flags & CAVE_WRAP) ? kBatchShapeErdWrap : kBatchShapeErd;
+}
+
+uint32_t tune_kind(const DevJob &j, uint32_t shape)
+{
+    if (shape == kBatchShapeCube) return j.flags & 7u;
+    return kTuneErd | (shape == kBatchShapeErdWrap ? 4u : 0u) | (j.flags & 3u);
+}
 
 uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z)
 {
@@ -237,10 +248,9 @@ DevJob make_job(const CaveSimSpec &s, uint32_t slot)
 int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPlan &plan)
 {
     plan = BatchPlan{};
-    plan.erd = n > 0 && (specs[0].flags & CAVE_SIM_SHAPE_ERD);
+    plan.shape = n > 0 ? spec_shape(specs[0]) : kBatchShapeCube;
     for (size_t i = 0; i < n; i++)
-        if (((specs[i].flags & CAVE_SIM_SHAPE_ERD) != 0) != plan.erd)
-            return fail(CAVE_E_INVALID, "a persistent launch runs one cell shape");
+        if (spec_shape(specs[i]) != plan.shape) return fail(CAVE_E_INVALID, "a persistent launch runs one cell shape");
     plan.jobs.resize(n);
     plan.ipt.resize(n);
     std::vector<uint64_t> words(n);
@@ -253,7 +263,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         words[i] = job_words(j.X, j.Y, j.Z);
         cost[i] = (double)j.X * j.Y * j.Z * j.ticks;
         key_bucket[i] = edge_bucket(j.X, j.Y, j.Z);
-        BucketWork &b = buckets[{key_bucket[i], tune_kind(j)}];
+        BucketWork &b = buckets[{key_bucket[i], tune_kind(j, plan.shape)}];
         if (b.work == 0)
             for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
         b.work += cost[i];
@@ -351,7 +361,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         }
     }
     plan.lds_bytes = (uint32_t)(lds_words * 4u);
-    const uint32_t per_cu = resident_per_cu(threads, plan.lds_bytes, plan.erd);
+    const uint32_t per_cu = resident_per_cu(threads, plan.lds_bytes, plan.shape);
     const uint32_t resident = cu_count() * per_cu;
 
     // ---- single-block time estimates, splitting, order ----
@@ -361,7 +371,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     for (size_t i = 0; i < n; i++) {
         const DevJob &j = plan.jobs[i];
         double r = 0;
-        ipt_of[i] = best_ipt(key_bucket[i], tune_kind(j), threads, &r);
+        ipt_of[i] = best_ipt(key_bucket[i], tune_kind(j, plan.shape), threads, &r);
         // a bucket's device-wide rate is shared by `resident` blocks
         t[i] = r > 0 ? cost[i] * resident / r : cost[i];
         total += t[i];
@@ -458,7 +468,7 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
     CAVE_HIP(hipEventCreate(&e0));
     CAVE_HIP(hipEventCreate(&e1));
     CAVE_HIP(hipEventRecord(e0, st));
-    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st, plan.nsync > 0, plan.erd);
+    rc = launch_batch(a, plan.blocks, plan.threads, plan.lds_bytes, st, plan.nsync > 0, plan.shape);
     if (rc) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

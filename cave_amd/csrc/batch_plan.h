@@ -29,13 +29,15 @@ struct BatchPlan {
     uint32_t nsync = 0;
     uint64_t final_words = 0;   // staging for final cells (CAVE_SIM_FINAL_HASH)
     double predicted_ms = 0;    // the tuner's estimate of the launch (0 without tuning)
-    bool erd = false;           // the jobs are of the ERD shape (a plan holds one shape)
+    uint32_t shape = kBatchShapeCube;  // the jobs' cell shape (a plan holds one shape)
 };
 
-// The rate table's neighbourhood key: F|E|C for cube jobs, kTuneErd | F|E
-// for jobs of the ERD shape, whose tick costs differ from the cube's.
+// The rate table's neighbourhood key: F|E|C for cube jobs; for the ERD
+// shape, whose ticks cost differently, kTuneErd | F|E, plus 4 with wrap.
 constexpr uint32_t kTuneErd = 16u;
-uint32_t tune_kind(const DevJob &j);
+uint32_t tune_kind(const DevJob &j, uint32_t shape);
+// The persistent kernel form (kBatchShape*) a spec runs on.
+uint32_t spec_shape(const CaveSimSpec &s);
 
 // Grid-edge bucket of a simulation (edge = cube root of its cell count):
 // [0,32) [32,48) [48,64) [64,96) [96,128) [128,...).
@@ -43,7 +45,7 @@ uint32_t edge_bucket(uint32_t X, uint32_t Y, uint32_t Z);
 
 // Blocks of `threads` threads one CU keeps resident with `lds_bytes` of
 // dynamic LDS each.
-uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, bool erd);
+uint32_t resident_per_cu(uint32_t threads, uint32_t lds_bytes, uint32_t shape);
 uint32_t cu_count();
 DevJob make_job(const CaveSimSpec &s, uint32_t slot);
 

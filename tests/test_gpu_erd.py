@@ -180,3 +180,32 @@ def test_server_refuses_erd_requests():
         with pytest.raises(cv.CaveError) as e:
             srv.submit(s, 1)
     assert e.value.code == cv.CAVE_E_UNSUPPORTED
+
+
+# wrapping ERD grids (W = 8): tiny axes (the same cell counts once per offset
+# that reaches it), rows of one word, rows ending on a word boundary, the LDS
+# and the global tier
+ERD_WRAP_BATCH = [((1, 1, 1), 1, 11, 6), ((2, 3, 2), 2, 9, 8), ((3, 2, 5), 3, 10, 8), ((9, 7, 6), 4, 11, 10),
+                  ((20, 5, 7), 2, 9, 10), ((64, 20, 12), 5, 11, 12), ((33, 32, 31), 3, 10, 12),
+                  ((40, 21, 17), 7, 11, 14), ((100, 81, 60), 2, 11, 10), ((96, 64, 40), 6, 9, 9)]
+
+
+@pytest.mark.parametrize("max_parts", [1, 3], ids=["whole", "3parts"])
+def test_erd_wrap_persistent_engine_matches_oracle(max_parts):
+    """Wrapping ERD grids on the persistent kernel's wrap form: rows y+-1 and
+    planes z+-1 past a face come from the opposite face, and x-neighbours
+    across the row ends from the row's other end; every per-tick row, grid
+    info and final hash against the oracle (which wraps every coordinate of
+    every offset, searchSimulationERD.slang with W)."""
+    sims = erd_sims(ERD_WRAP_BATCH, 12)
+    print(f"erd wrap persistent max_parts={max_parts}", flush=True)
+    res, rows, _ = cv.batch_run(sims, engine=cv.ENGINE_PERSISTENT, per_tick=True, max_parts=max_parts)
+    for i, (s, r) in enumerate(zip(sims, res)):
+        X, Y, Z = s.dims
+        init = O.init_cells_erd(X, Y, Z, s.M, s.spawn, s.spawn_mode, s.density, s.seed)
+        t, info, orows, final = O.run_rule_erd(X, Y, Z, s.M, s.flags, s.birth, s.survival, s.ticks, init)
+        assert r.ticks_run == t, f"sim {i}"
+        for f, _ in O.GridInfo._fields_:
+            assert getattr(r.info, f) == getattr(info, f), (i, f)
+        assert [x.as_tuple() for x in rows[i]] == [x.as_tuple() for x in orows], f"sim {i}"
+        assert r.final_hash == O.erd_hash(final, X, Y, Z), f"sim {i}"
