@@ -292,7 +292,9 @@ int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *resu
  * a JSON array of {edge_bucket, flags, threads, items_per_thread,
  * cell_updates_per_s}: one calibration launch per (grid-edge bucket [0,32)
  * [32,48) [48,64) [64,96) [96,128) [128,..), neighbourhood, block size, items
- * per thread).  No reference counterpart. */
+ * per thread).  `flags` is F|E|C for cube grids; ERD grids, whose ticks cost
+ * differently, are keyed 16 + F|E, plus 4 when they wrap.  No reference
+ * counterpart. */
 int cave_batch_tuning_report(char *buf, size_t cap);
 
 /* ================================================================== */
