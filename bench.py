@@ -213,6 +213,37 @@ def cpu_step_baseline(gpu_cells=None, gpu_ticks: int = 0, gpu_last=None, target_
     return base, parity
 
 
+def active_region(specs, res, dev_ms, nominal, tm, label):
+    """What the active region saved on a timed batch: the cells the ticks
+    computed (each tick computes the cells within one cell of a cell live so
+    far; CaveSimTiming.computed_cells) against the nominal X*Y*Z per tick, the
+    HBM-equivalent fraction of the computed cells, and the same list rerun
+    with CAVE_SIM_FULL_GRID (every cell every tick), whose results must be
+    equal."""
+    import numpy as np
+
+    import cave_amd as cv
+
+    computed = int(tm["computed_cells"].astype(np.int64).sum())
+    full = np.array(specs, copy=True)
+    full["sim_flags"] |= cv.SIM_FULL_GRID
+    fres, fms = cv.batch_run_np(full, engine=cv.ENGINE_PERSISTENT)
+    same = all(np.array_equal(fres[f], res[f]) for f in
+               ("died", "born", "ticks_survived", "status", "max_alive", "final_alive", "classification",
+                "ticks_run", "last"))
+    if not same:
+        log(f"{label}: full-grid and active-region results differ")
+    peak = HBM_PEAK_GBS * 1e9
+    return {"computed_cells": computed, "computed_fraction": computed / nominal,
+            "computed_cell_updates_per_s": computed / (dev_ms / 1e3),
+            "roofline_frac_computed": computed / (dev_ms / 1e3) / peak,
+            "full_grid": {"device_ms": fms, "speedup_of_region": fms / dev_ms,
+                          "roofline_frac": nominal / (fms / 1e3) / peak, "results_equal": same},
+            "note": "a tick computes only the cells within one cell of a cell live so far (the others are 0 "
+                    "and stay 0); value and cell_updates count every cell, X*Y*Z per tick, as the reference "
+                    "computes them; computed_* count the cells computed"}
+
+
 def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     """Rule-search rate at 49^3 x 20 ticks (FE, C in 2..5, <= 2 + 2 rule bits,
     32 seeds per config).  Persistent batched engine over the whole list; the
@@ -226,7 +257,7 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     cells = 49 ** 3
     cv.batch_run_np(specs[: seeds * 8])  # warm-up (code objects, allocator)
     t0 = time.perf_counter()
-    res, dev_ms = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT)
+    res, dev_ms, tm, _ = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
     wall = time.perf_counter() - t0
     agg = W.aggregate_by_config(specs, res, seeds)
     # one aggregate JSONL record per rule config (RuleAggregator,
@@ -261,7 +292,9 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
                             "note": "one RuleAggregator record per distinct rule (configs drawing the same "
                                     "rule fold together, as in the reference); written after the timed launch"},
     }
-    log(f"  persistent engine {dev_ms:.2f} ms; plain engine on {plain_configs} configs")
+    out["active_region"] = active_region(specs, res, dev_ms, updates, tm, "search")
+    log(f"  persistent engine {dev_ms:.2f} ms (every cell: {out['active_region']['full_grid']['device_ms']:.2f} ms); "
+        f"plain engine on {plain_configs} configs")
     sub = specs[: plain_configs * seeds]
     pres, pms = cv.batch_run_np(sub, engine=cv.ENGINE_PLAIN)
     assert np.array_equal(pres["ticks_run"], res["ticks_run"][: len(sub)]), "plain / persistent disagree"
@@ -353,8 +386,9 @@ def run_erd_search_bench(n_configs=4096, seeds=32, plain_configs=64):
     specs = W.search_uniform_erd(n_configs, seeds)
     cells = 49 ** 3
     cv.batch_run_np(specs[: seeds * 8])  # warm-up
-    res, dev_ms = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT)
+    res, dev_ms, tm, _ = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
     updates = int(res["ticks_run"].astype(np.int64).sum()) * cells
+    region = active_region(specs, res, dev_ms, updates, tm, "search_erd")
     sub = specs[: plain_configs * seeds]
     pres, pms = cv.batch_run_np(sub, engine=cv.ENGINE_PLAIN)
     same = bool(np.array_equal(pres["ticks_run"], res["ticks_run"][: len(sub)]) and
@@ -365,6 +399,7 @@ def run_erd_search_bench(n_configs=4096, seeds=32, plain_configs=64):
             "value": n_configs / (dev_ms / 1e3), "unit": "configs/s", "configs": n_configs,
             "seeds_per_config": seeds, "device_ms": dev_ms, "cell_updates_per_s": updates / (dev_ms / 1e3),
             "hbm_equivalent_frac": updates / (dev_ms / 1e3) / (HBM_PEAK_GBS * 1e9),
+            "active_region": region,
             "plain": {"value": plain_configs / (pms / 1e3), "unit": "configs/s", "device_ms": pms,
                       "sample": f"first {plain_configs} configs x {seeds} seeds, the reference-form ERD kernel: "
                                 "one step + one manager launch per sim per tick",
@@ -404,6 +439,7 @@ def run_uneven_bench(n_configs=2048, cpu=True):
     edge = np.cbrt(cells.astype(np.float64))
     bidx = np.digitize(edge, [32, 48, 64, 96, 128])
     block_s = (tm["end_us"] - tm["start_us"]) * tm["blocks"] / 1e6
+    comp = tm["computed_cells"].astype(np.int64)
     buckets = []
     for b in range(6):
         for fl in (1, 3, 7):
@@ -415,7 +451,9 @@ def run_uneven_bench(n_configs=2048, cpu=True):
                             "cell_updates": int(ups[m].sum()), "block_ms": float(block_s[m].sum() * 1e3),
                             "split_configs": int((tm["blocks"][m] > 1).sum()),
                             "items_per_thread": int(np.bincount(tm["items_per_thread"][m]).argmax()),
-                            "cell_updates_per_block_s": rate, "achieved_frac": rate * 1.0 / block_share})
+                            "cell_updates_per_block_s": rate, "achieved_frac": rate * 1.0 / block_share,
+                            "computed_fraction": float(comp[m].sum() / ups[m].sum()),
+                            "achieved_frac_computed": comp[m].sum() / block_s[m].sum() / block_share})
     out = {
         "metric": "rule-configs/sec, uneven sizes (edge 24..160, F/FE/FEV, C 2..16, 20..200 ticks)",
         "value": n_configs / (dev_ms / 1e3), "value_wall": n_configs / wall, "unit": "configs/s",
@@ -427,7 +465,8 @@ def run_uneven_bench(n_configs=2048, cpu=True):
         "last_finish_ms": float(tm["end_us"].max() / 1e3), "p50_finish_ms": float(np.percentile(tm["end_us"], 50) / 1e3),
         "buckets": buckets,
     }
-    log(f"  timed call {dev_ms:.2f} ms")
+    out["active_region"] = active_region(specs, res, dev_ms, int(ups.sum()), tm, "search_uneven")
+    log(f"  timed call {dev_ms:.2f} ms (every cell: {out['active_region']['full_grid']['device_ms']:.2f} ms)")
     if cpu:
         O = oracle_module()
         n = cpu_threads()

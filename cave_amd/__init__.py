@@ -23,7 +23,7 @@ FACES, EDGES, CORNERS, WRAP = 1, 2, 4, 8
 SPAWN_RANDOM, SPAWN_FILLED, SPAWN_BUFFER, SPAWN_RENDER_RANDOM = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_STREAM, KERNEL_STREAM_NIBBLE, KERNEL_ERD = 0, 1, 2, 3, 4
 ENGINE_AUTO, ENGINE_PERSISTENT, ENGINE_PLAIN = 0, 1, 2
-SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH, SIM_SHAPE_ERD = 1, 2, 4, 8
+SIM_NO_EARLY_EXIT, SIM_FINAL_HASH, SIM_LEGACY_DEATH, SIM_SHAPE_ERD, SIM_FULL_GRID = 1, 2, 4, 8, 16
 SHAPE_CUBE, SHAPE_ERD = 0, 1
 
 
@@ -390,13 +390,14 @@ RESULT_DTYPE = np.dtype({
     "itemsize": 88})
 
 
-TIMING_DTYPE = np.dtype([("start_us", "<f8"), ("end_us", "<f8"), ("blocks", "<u4"), ("items_per_thread", "<u4")])
+TIMING_DTYPE = np.dtype([("start_us", "<f8"), ("end_us", "<f8"), ("blocks", "<u4"), ("items_per_thread", "<u4"),
+                         ("computed_cells", "<u8")])
 
 
 def batch_run_np(specs: np.ndarray, engine=ENGINE_AUTO, block_threads=0, max_parts=0, timing=False):
     """Evaluates a SIM_DTYPE array; returns (RESULT_DTYPE array, device ms),
     plus a TIMING_DTYPE array (device start / end microseconds, blocks, items
-    per thread per simulation) and the block size used when timing is set
+    per thread, cells computed per simulation) and the block size used when timing is set
     (persistent engine)."""
     specs = np.ascontiguousarray(specs, dtype=SIM_DTYPE)
     res = np.zeros(len(specs), dtype=RESULT_DTYPE)

@@ -56,7 +56,8 @@ int validate_sim(const CaveSimSpec &s)
     if (s.max_cell_state < 1 || s.max_cell_state > 16) return fail(CAVE_E_INVALID, "max_cell_state must be 1..16");
     // W: the cube search kernel ignores it (searchSimulationCube.slang:26-29), the ERD one wraps
     if (s.neighborhood_flags & ~15u) return fail(CAVE_E_INVALID, "bad neighborhood flags");
-    if (s.flags & ~(CAVE_SIM_NO_EARLY_EXIT | CAVE_SIM_FINAL_HASH | CAVE_SIM_LEGACY_DEATH | CAVE_SIM_SHAPE_ERD))
+    if (s.flags & ~(CAVE_SIM_NO_EARLY_EXIT | CAVE_SIM_FINAL_HASH | CAVE_SIM_LEGACY_DEATH | CAVE_SIM_SHAPE_ERD |
+                    CAVE_SIM_FULL_GRID))
         return fail(CAVE_E_INVALID, "unknown sim flags");
     if (s.ticks == 0) return fail(CAVE_E_INVALID, "ticks must be >= 1");
     if (s.spawn_mode == CAVE_SPAWN_BUFFER || s.spawn_mode > CAVE_SPAWN_RENDER_RANDOM)

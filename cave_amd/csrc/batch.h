@@ -52,6 +52,10 @@ struct JobSync {
     // 0-23) | born (24-47) | arrivals (48-63), one slot per tick mod 3
     uint64_t tick[3];
     uint64_t pad2_;
+    // those jobs' per-tick bounding boxes, read one tick later: slot tick % 4,
+    // encoded as nmin / xmax1
+    int32_t lag_nmin[4][4];
+    int32_t lag_xmax1[4][4];
 };
 
 // Result record written by the kernel (host converts to CaveSimResult).
@@ -63,6 +67,7 @@ struct DevResult {
     uint64_t t_start, t_end;  // device wall clock at claim / at completion
     uint32_t err;             // 1: a part rendezvous timed out
     uint32_t pad_;
+    uint64_t computed;        // cells of the active regions of its ticks (every part's)
 };
 
 struct BatchArgs {

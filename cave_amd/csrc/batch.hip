@@ -448,17 +448,65 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
     }
 }
 
-template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE>
-__device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
-                           const RuleConsts &rc, StatAcc &acc)
+// The cells a tick computes: an inclusive box of cell coordinates (empty when
+// hi < lo on an axis).  Every cell outside it is 0 before the tick and has no
+// live neighbour, so it stays 0 and counts as dead; see run_entry.
+// Its items for this part (plan_region): word columns w0 .. w0+nw-1 of rows
+// y0 .. y0+ny-1, planes [za, zb) cut into z-chunks of tz planes, n items.
+// cells: the region's cells over the whole job (every part's planes, whole
+// words in x), for the computed-cells count.
+struct Region {
+    int32_t lo[3], hi[3];
+    uint32_t w0, nw, y0, ny, za, zb, tz, n;
+    uint64_t cells;
+};
+
+// ceil(a / b), b >= 1, from a float quotient corrected to the exact value:
+// a few instructions instead of the integer division's ~35 on the one thread
+// that plans the next tick.  For the item and plane counts it sees (a < 2^26)
+// the estimate is within one of the result, so each loop runs at most once.
+__device__ __forceinline__ uint32_t div_ceil_small(uint32_t a, uint32_t b)
 {
-    for (uint32_t it = threadIdx.x; it < g.nitems; it += blockDim.x) {
-        const int w = (int)(it % g.W);
-        const uint32_t rest = it / g.W;
-        const int y = (int)(rest % g.Y);
-        const uint32_t zc = g.zc0 + rest / g.Y;
-        const int z0 = (int)(zc * g.tzc);
-        const int z1 = min(z0 + (int)g.tzc, (int)g.Z);
+    uint32_t q = (uint32_t)__builtin_ceilf((float)a * __builtin_amdgcn_rcpf((float)b));
+    while (q * b < a) q++;
+    while (q > 0u && (q - 1u) * b >= a) q--;
+    return q;
+}
+
+// The items of a region on this part's planes: (word column, z-chunk) pairs,
+// with the z-chunks cut so that the tick has about as many items as the plan
+// gave the whole part (thread 0 computes it once per tick).
+__device__ __forceinline__ void plan_region(Region &r, const JobGeo &g)
+{
+    const int za = max(r.lo[2], (int)g.zlo), zb = min(r.hi[2] + 1, (int)g.zhi);
+    r.n = 0u;
+    r.za = r.zb = (uint32_t)max(za, 0);
+    r.w0 = r.nw = r.y0 = r.ny = 0u;
+    r.tz = 1u;
+    if (za >= zb || r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1]) return;
+    r.w0 = (uint32_t)r.lo[0] >> 5;
+    r.nw = ((uint32_t)r.hi[0] >> 5) - r.w0 + 1u;
+    r.y0 = (uint32_t)r.lo[1];
+    r.ny = (uint32_t)(r.hi[1] - r.lo[1] + 1);
+    r.za = (uint32_t)za;
+    r.zb = (uint32_t)zb;
+    const uint32_t nzp = r.zb - r.za, cols = r.nw * r.ny;
+    const uint32_t nzc = min(nzp, max(1u, div_ceil_small(g.nitems, cols)));
+    r.tz = div_ceil_small(nzp, nzc);
+    r.n = cols * div_ceil_small(nzp, r.tz);
+}
+
+// One tick over the region's items (plan_region).
+template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE>
+__device__ __forceinline__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
+                           const RuleConsts &rc, const Region &r, StatAcc &acc)
+{
+    for (uint32_t it = threadIdx.x; it < r.n; it += blockDim.x) {
+        const int w = (int)(r.w0 + it % r.nw);
+        const uint32_t rest = it / r.nw;
+        const int y = (int)(r.y0 + rest % r.ny);
+        const int z0 = (int)(r.za + (rest / r.ny) * r.tz);
+        const int z1 = min(z0 + (int)r.tz, (int)r.zb);
         if constexpr (SHAPE != kBatchShapeCube)
             run_item_erd<FLAGS, COH, GLOBAL, SHAPE == kBatchShapeErdWrap>(g, S, acur, anext, rc, w, y, z0, z1, acc);
         else run_item<FLAGS, COH, GLOBAL>(g, S, acur, anext, rc, w, y, z0, z1, acc);
@@ -469,27 +517,27 @@ __device__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acu
 // parameter (a kernel instantiation per shape): with both shapes' tick code in
 // one kernel this compiler fails ("illegal VGPR to SGPR copy").
 template <bool COH, bool GLOBAL, uint32_t SHAPE>
-__device__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
-                         const RuleConsts &rc, StatAcc &acc)
+__device__ __forceinline__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
+                         const RuleConsts &rc, const Region &r, StatAcc &acc)
 {
     if constexpr (SHAPE != kBatchShapeCube) {
         switch (flags & 3u) {
-        case 1: tick_items<1, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
-        case 2: tick_items<2, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
-        case 3: tick_items<3, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
-        default: tick_items<0, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, acc); break;
+        case 1: tick_items<1, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
+        case 2: tick_items<2, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
+        case 3: tick_items<3, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
+        default: tick_items<0, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
         }
         return;
     }
     switch (flags & 7u) {
-    case 1: tick_items<1, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 2: tick_items<2, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 3: tick_items<3, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 4: tick_items<4, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 5: tick_items<5, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 6: tick_items<6, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    case 7: tick_items<7, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
-    default: tick_items<0, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, acc); break;
+    case 1: tick_items<1, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 2: tick_items<2, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 3: tick_items<3, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 4: tick_items<4, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 5: tick_items<5, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 6: tick_items<6, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 7: tick_items<7, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    default: tick_items<0, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
     }
 }
 
@@ -555,49 +603,128 @@ __device__ void store_final(const JobGeo &g, const uint32_t *S, uint32_t *out)
     }
 }
 
-// Block sums of three counters; wave 0 (thread 0 among it) receives them.
-// scratch: 3 words per wave.
-__device__ __forceinline__ void block_sum3(uint32_t a, uint32_t b, uint32_t c, uint32_t *scratch, uint32_t out[3])
+// A box as three words of two 16-bit fields, every field reduced by a
+// minimum: w0 = min x | min y << 16, w1 = min z | ~max x << 16,
+// w2 = ~max y | ~max z << 16, with ~m = 0xFFFE - m.  Coordinates are below
+// 4096; an empty box (min 0x7fffffff, max -1) is all 0xFFFF.
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b)
+{
+    typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b)));
+}
+
+__device__ __forceinline__ void box_pack(const int32_t bmin[3], const int32_t bmax[3], uint32_t w[3])
+{
+    uint32_t f[6];
+    for (int k = 0; k < 3; k++) {
+        f[k] = min((uint32_t)bmin[k], 0xFFFFu);
+        f[3 + k] = 0xFFFEu - (uint32_t)bmax[k];
+    }
+    w[0] = f[0] | f[1] << 16;
+    w[1] = f[2] | f[3] << 16;
+    w[2] = f[4] | f[5] << 16;
+}
+
+__device__ __forceinline__ void box_unpack(const uint32_t w[3], int32_t bmin[3], int32_t bmax[3])
+{
+    const uint32_t f[6] = {w[0] & 0xFFFFu, w[0] >> 16, w[1] & 0xFFFFu, w[1] >> 16, w[2] & 0xFFFFu, w[2] >> 16};
+    for (int k = 0; k < 3; k++) {
+        bmin[k] = f[k] == 0xFFFFu ? 0x7fffffff : (int32_t)f[k];
+        bmax[k] = (int32_t)0xFFFEu - (int32_t)f[3 + k];  // 0xFFFF: -1
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_pk_min(uint32_t v)
+{
+    return (uint32_t)wave_reduce_dpp((int32_t)v, -1, [](int32_t x, int32_t y) {
+        return (int32_t)pk_min_u16((uint32_t)x, (uint32_t)y);
+    });
+}
+
+// Block min / max of the bounding box; wave 0 (thread 0 among it) receives
+// it.  scratch: 3 words per wave.
+__device__ __forceinline__ void block_bbox(const StatAcc &acc, uint32_t *scratch, int32_t bmin[3], int32_t bmax[3])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    a = wave_sum_u32(a);
-    b = wave_sum_u32(b);
-    c = wave_sum_u32(c);
+    uint32_t w[3];
+    box_pack(acc.bmin, acc.bmax, w);
+    for (int k = 0; k < 3; k++) w[k] = wave_pk_min(w[k]);
+    if (lane == 0)
+        for (int k = 0; k < 3; k++) scratch[3 * wave + k] = w[k];
+    __syncthreads();
+    if (wave == 0) {  // lane i reduces wave i's partials
+        const bool has = lane < nw;
+        const uint32_t *p = scratch + 3 * (has ? lane : 0);
+        for (int k = 0; k < 3; k++) w[k] = wave_pk_min(has ? p[k] : 0xFFFFFFFFu);
+        box_unpack(w, bmin, bmax);
+    }
+}
+
+// Block sums of the dying, born and surviving counters and the block's box,
+// with one barrier: wave 0 (thread 0 among it) receives them.  scratch: 3 + 3
+// words per wave.
+__device__ __forceinline__ void block_stats(const StatAcc &acc, uint32_t *ssum, uint32_t *sbox, uint32_t tot[3],
+                                            int32_t bmin[3], int32_t bmax[3])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const uint32_t a = wave_sum_u32(acc.dying), b = wave_sum_u32(acc.born), c = wave_sum_u32(acc.surv);
+    uint32_t w[3];
+    box_pack(acc.bmin, acc.bmax, w);
+    for (int k = 0; k < 3; k++) w[k] = wave_pk_min(w[k]);
     if (lane == 0) {
-        scratch[3 * wave] = a;
-        scratch[3 * wave + 1] = b;
-        scratch[3 * wave + 2] = c;
+        ssum[3 * wave] = a;
+        ssum[3 * wave + 1] = b;
+        ssum[3 * wave + 2] = c;
+        for (int k = 0; k < 3; k++) sbox[3 * wave + k] = w[k];
     }
     __syncthreads();
     if (wave == 0) {  // lane i reduces wave i's partials (nw <= 16)
         const bool has = lane < nw;
-        const uint32_t *v = scratch + 3 * (has ? lane : 0);
-        out[0] = wave_sum_u32(has ? v[0] : 0u);
-        out[1] = wave_sum_u32(has ? v[1] : 0u);
-        out[2] = wave_sum_u32(has ? v[2] : 0u);
+        const uint32_t *q = ssum + 3 * (has ? lane : 0);
+        const uint32_t *p = sbox + 3 * (has ? lane : 0);
+        for (int k = 0; k < 3; k++) {
+            tot[k] = wave_sum_u32(has ? q[k] : 0u);
+            w[k] = wave_pk_min(has ? p[k] : 0xFFFFFFFFu);
+        }
+        box_unpack(w, bmin, bmax);
     }
 }
 
-// Block min / max of the bounding box; wave 0 (thread 0 among it) receives
-// it.  scratch: 6 words per wave.
-__device__ __forceinline__ void block_bbox(const StatAcc &acc, int32_t *scratch, int32_t bmin[3], int32_t bmax[3])
+// The region of the next tick (thread 0): the union u of the live boxes
+// known so far, grown by `grow` cells and clipped to the grid, or the whole
+// grid when `full`; re-planned only when it changed.  `fresh`: r holds no
+// region yet.
+__device__ __forceinline__ void set_region(Region &r, const int32_t ulo[3], const int32_t uhi[3], int grow,
+                                           const DevJob &j, const JobGeo &g, bool full, bool fresh)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    int32_t v[6];
+    const int32_t dim[3] = {(int32_t)j.X, (int32_t)j.Y, (int32_t)j.Z};
+    bool empty = false;
+    for (int k = 0; k < 3; k++) empty |= uhi[k] < ulo[k];
+    int32_t lo[3], hi[3];
+    bool same = !fresh;
     for (int k = 0; k < 3; k++) {
-        v[k] = wave_min_i32(acc.bmin[k]);
-        v[3 + k] = wave_max_i32(acc.bmax[k]);
-    }
-    if (lane == 0)
-        for (int k = 0; k < 6; k++) scratch[6 * wave + k] = v[k];
-    __syncthreads();
-    if (wave == 0) {  // lane i reduces wave i's partials
-        const bool has = lane < nw;
-        const int32_t *p = scratch + 6 * (has ? lane : 0);
-        for (int k = 0; k < 3; k++) {
-            bmin[k] = wave_min_i32(has ? p[k] : 0x7fffffff);
-            bmax[k] = wave_max_i32(has ? p[3 + k] : -1);
+        if (full) {
+            lo[k] = 0;
+            hi[k] = dim[k] - 1;
+        } else if (empty) {
+            lo[k] = 0;
+            hi[k] = -1;
+        } else {
+            lo[k] = max(0, ulo[k] - grow);
+            hi[k] = min(dim[k] - 1, uhi[k] + grow);
         }
+        same = same && lo[k] == r.lo[k] && hi[k] == r.hi[k];
+    }
+    if (same) return;
+    for (int k = 0; k < 3; k++) {
+        r.lo[k] = lo[k];
+        r.hi[k] = hi[k];
+    }
+    plan_region(r, g);
+    r.cells = 0u;
+    if (!empty || full) {
+        const uint32_t x0 = 32u * ((uint32_t)lo[0] >> 5), x1 = min(j.X, 32u * (((uint32_t)hi[0] >> 5) + 1u));
+        r.cells = (uint64_t)(x1 - x0) * (uint32_t)(hi[1] - lo[1] + 1) * (uint32_t)(hi[2] - lo[2] + 1);
     }
 }
 
@@ -629,7 +756,7 @@ __device__ __forceinline__ bool parts_all_claimed(const BatchArgs &a, uint64_t n
 // parts_all_claimed), a wait longer than `wait_limit` aborts the job for all
 // parts.  Returns false (to every thread) when the job was aborted.
 template <bool SERVE>
-__device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
+__device__ __forceinline__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
                                 const uint32_t part_sum[3], const int32_t bmin[3], const int32_t bmax[3],
                                 uint32_t tot[3], int32_t tmin[3], int32_t tmax[3], uint32_t *s_flag)
 {
@@ -695,22 +822,47 @@ __device__ bool part_rendezvous(const BatchArgs &a, const DevJob &j, uint64_t ne
 // atomics, then arrives, polls and reads the totals back: two more round
 // trips per tick.)  The slot of phase + 1 (= phase - 2) was last read before
 // every part arrived for phase - 1: part 0 clears it before it arrives.
+//
+// The same rendezvous also passes the parts' bounding boxes on, one tick late:
+// each part max-combines its box of this tick into slot phase % 4 right after
+// arriving (the atomics complete before its next arrival, at the wait for its
+// stores that opens the next rendezvous), and once every part has arrived it
+// reads the job's box of the previous tick from slot (phase - 1) % 4 into
+// pmin / pmax (empty for phase 1).  Part 0 clears slot (phase + 1) % 4, last
+// read before every part arrived for phase - 1, before it arrives.
 template <bool SERVE>
-__device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
-                                     uint32_t born, uint32_t surv, uint32_t *born_tot, uint32_t *surv_tot,
+__device__ __forceinline__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint64_t need, uint32_t phase,
+                                     uint32_t born, uint32_t surv, const int32_t bmin[3], const int32_t bmax[3],
+                                     uint32_t *born_tot, uint32_t *surv_tot, int32_t pmin[3], int32_t pmax[3],
                                      uint32_t *s_flag)
 {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         JobSync *js = a.sync + j.sync_slot;
-        if (j.part == 0u && phase >= 2u) {
-            __hip_atomic_store(&js->tick[(phase + 1u) % 3u], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (j.part == 0u) {
+            if (phase >= 2u)
+                __hip_atomic_store(&js->tick[(phase + 1u) % 3u], (uint64_t)0, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t ls = (phase + 1u) % 4u;
+            for (int k = 0; k < 3; k++) {
+                __hip_atomic_store(&js->lag_nmin[ls][k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&js->lag_xmax1[ls][k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             __builtin_amdgcn_s_waitcnt(0);
         }
         uint64_t *w = &js->tick[phase % 3u];
         const uint64_t add = (uint64_t)surv | ((uint64_t)born << 24) | ((uint64_t)1 << 48);
         uint64_t v = __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
+        {
+            const uint32_t ls = phase % 4u;
+            for (int k = 0; k < 3; k++) {
+                __hip_atomic_fetch_max(&js->lag_nmin[ls][k], 0x7fffffff - bmin[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_max(&js->lag_xmax1[ls][k], bmax[k] + 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         uint64_t t0 = wall_clock64();
         bool started = false;
         uint32_t ok = 1u;
@@ -727,6 +879,18 @@ __device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint64
         }
         *surv_tot = (uint32_t)(v & 0xFFFFFFu);
         *born_tot = (uint32_t)((v >> 24) & 0xFFFFFFu);
+        for (int k = 0; k < 3; k++) {
+            pmin[k] = 0x7fffffff;
+            pmax[k] = -1;
+        }
+        if (ok && phase >= 2u) {
+            const uint32_t ls = (phase - 1u) % 4u;
+            for (int k = 0; k < 3; k++) {
+                pmin[k] = 0x7fffffff -
+                          __hip_atomic_load(&js->lag_nmin[ls][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pmax[k] = __hip_atomic_load(&js->lag_xmax1[ls][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
+            }
+        }
         *s_flag = ok;
     }
     __syncthreads();
@@ -736,7 +900,12 @@ __device__ bool part_rendezvous_tick(const BatchArgs &a, const DevJob &j, uint64
 // Block-wide scratch of the batched kernels.
 struct BlockShared {
     uint32_t sum[3 * 16];
-    int32_t box[6 * 16];
+    uint32_t box[3 * 16];
+    Region reg;        // the cells of the next tick (thread 0 writes it between ticks)
+    int32_t ulo[3], uhi[3];  // the union of the live boxes so far (thread 0)
+    uint64_t computed;       // cells of the regions so far, whole job (thread 0)
+    CaveGridInfo info;       // the job's manager state (thread 0)
+    CaveTickStats last;      // its last tick's row (thread 0)
     uint32_t done;
     uint32_t ok;
     uint64_t claim;
@@ -752,7 +921,7 @@ struct BlockShared {
 // FAST_SPLIT: split jobs may take the one-word per-tick rendezvous.
 // SHAPE: the jobs' cell shape (kBatchShape*; make_job keeps F|E of ERD jobs).
 template <bool SERVE, bool FAST_SPLIT, uint32_t SHAPE>
-__device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
+__device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, uint32_t *arena, BlockShared &sh)
 {
     const uint64_t t_start = wall_clock64();
     const JobGeo g = job_geo(j);
@@ -773,12 +942,38 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
     if (j.tier == kTierLds) init_cells<false>(j, g, SL, acur);
     else if (multi) init_cells<true>(j, g, SG, acur);
     else init_cells<false>(j, g, SG, acur);
-    CaveGridInfo info{};
-    CaveTickStats last{};
+    // the manager's state and the last tick's row: thread 0's, kept in LDS
+    CaveGridInfo &info = sh.info;
+    CaveTickStats &last = sh.last;
+    if (threadIdx.x == 0) {
+        info = CaveGridInfo{};
+        last = CaveTickStats{};
+    }
     const uint32_t cells = j.X * j.Y * j.Z;
     uint32_t t3[3] = {0u, 0u, 0u};
     int32_t b3[3] = {0, 0, 0}, c3[3] = {0, 0, 0};
     bool ok = true;
+    // Active region (exact compute-skip).  A cell that is 0 and has no live
+    // neighbour stays 0 and counts as dead, so a tick need only compute the
+    // cells that are not 0 or have a live neighbour; the stores it leaves out
+    // would have written the values already there.  Every non-zero cell lies
+    // in the union U of the boxes of the live cells of the ticks so far (the
+    // initial box included): a cell is non-zero only for the M - 1 ticks
+    // after it was live.  A tick therefore computes U grown by one cell.  The
+    // live plane a tick writes was last written two ticks earlier, inside U,
+    // so no stale live bit is left outside what it rewrites.  Split jobs that
+    // take the one-word rendezvous learn the job's box one tick late; as a
+    // box grows by at most a cell per tick, U grown by two cells covers them.
+    // ERD jobs with wrap (and CAVE_SIM_FULL_GRID) compute every cell.
+    const bool full = SHAPE == kBatchShapeErdWrap || (j.sim_flags & CAVE_SIM_FULL_GRID) != 0u;
+    if (threadIdx.x == 0) {  // U, kept in LDS between ticks
+        const uint32_t dim[3] = {j.X, j.Y, j.Z};
+        for (int k = 0; k < 3; k++) {
+            sh.ulo[k] = (int32_t)j.init.box_lo[k];
+            sh.uhi[k] = (int32_t)min(j.init.box_hi[k], dim[k]) - 1;
+        }
+        set_region(sh.reg, sh.ulo, sh.uhi, 1, j, g, full, true);
+    }
     if (multi) ok = part_rendezvous<SERVE>(a, j, need, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
     // Split jobs whose per-tick totals fit one word take the one-atomic
@@ -792,30 +987,56 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
                  j.parts < (1u << 16);
     uint32_t part_dying_all = 0u, part_dying_last = 0u;  // thread 0, packed jobs
+    if (threadIdx.x == 0) sh.computed = 0u;
     uint32_t ticks_run = 0;  // the same in every thread and every part
     StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
         acc = StatAcc{};
-        if (j.tier == kTierLds) run_tick<false, false, SHAPE>(j.flags, g, SL, acur, anext, rc, acc);
-        else if (multi) run_tick<true, true, SHAPE>(j.flags, g, SG, acur, anext, rc, acc);
-        else run_tick<false, true, SHAPE>(j.flags, g, SG, acur, anext, rc, acc);
+        Region reg;  // the same in every thread: scalar registers
+        reg.w0 = __builtin_amdgcn_readfirstlane(sh.reg.w0);
+        reg.nw = __builtin_amdgcn_readfirstlane(sh.reg.nw);
+        reg.y0 = __builtin_amdgcn_readfirstlane(sh.reg.y0);
+        reg.ny = __builtin_amdgcn_readfirstlane(sh.reg.ny);
+        reg.za = __builtin_amdgcn_readfirstlane(sh.reg.za);
+        reg.zb = __builtin_amdgcn_readfirstlane(sh.reg.zb);
+        reg.tz = __builtin_amdgcn_readfirstlane(sh.reg.tz);
+        reg.n = __builtin_amdgcn_readfirstlane(sh.reg.n);
+        if (threadIdx.x == 0) sh.computed += sh.reg.cells;
+        if (j.tier == kTierLds) run_tick<false, false, SHAPE>(j.flags, g, SL, acur, anext, rc, reg, acc);
+        else if (multi) run_tick<true, true, SHAPE>(j.flags, g, SG, acur, anext, rc, reg, acc);
+        else run_tick<false, true, SHAPE>(j.flags, g, SG, acur, anext, rc, reg, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
         uint32_t tot[3];
         int32_t bmin[3], bmax[3];
-        block_sum3(acc.dying, acc.born, acc.surv, sh.sum, tot);
+        block_stats(acc, sh.sum, sh.box, tot, bmin, bmax);
         if (packed) {
+            int32_t pmin[3], pmax[3];
             if (threadIdx.x == 0) {
                 part_dying_all += tot[0];
                 part_dying_last = tot[0];
             }
-            ok = part_rendezvous_tick<SERVE>(a, j, need, t + 1u, tot[1], tot[2], &tot[1], &tot[2], &sh.ok);
+            ok = part_rendezvous_tick<SERVE>(a, j, need, t + 1u, tot[1], tot[2], bmin, bmax, &tot[1], &tot[2], pmin,
+                                             pmax, &sh.ok);
             if (!ok) break;
-        } else if (multi) {
+            if (threadIdx.x == 0) {  // the job's box of the previous tick
+                for (int k = 0; k < 3; k++) {
+                    sh.ulo[k] = min(sh.ulo[k], pmin[k]);
+                    sh.uhi[k] = max(sh.uhi[k], pmax[k]);
+                }
+                set_region(sh.reg, sh.ulo, sh.uhi, 2, j, g, full, false);
+            }
+        } else {
             // every tick: whether this tick is the last is known only after the totals
-            block_bbox(acc, sh.box, bmin, bmax);
-            ok = part_rendezvous<SERVE>(a, j, need, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
+            if (multi) ok = part_rendezvous<SERVE>(a, j, need, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
             if (!ok) break;
+            if (threadIdx.x == 0) {  // the job's box of this tick
+                for (int k = 0; k < 3; k++) {
+                    sh.ulo[k] = min(sh.ulo[k], bmin[k]);
+                    sh.uhi[k] = max(sh.uhi[k], bmax[k]);
+                }
+                set_region(sh.reg, sh.ulo, sh.uhi, 1, j, g, full, false);
+            }
         }
         if (threadIdx.x == 0) {
             // packed jobs learn the dying total after the last tick: it is
@@ -843,8 +1064,6 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
             sh.done = stop ? 1u : 0u;
         }
         __syncthreads();
-        // the bounding box is reduced only when someone reads it
-        if (!multi && (sh.done || a.per_tick)) block_bbox(acc, sh.box, bmin, bmax);
         if (threadIdx.x == 0 && !packed && (sh.done || a.per_tick)) {
             for (int k = 0; k < 3; k++) {
                 last.bbox_min[k] = bmin[k];
@@ -891,7 +1110,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         else __syncthreads();
     }
     if (threadIdx.x == 0 && j.part == 0u) {
-        DevResult r;
+        DevResult &r = a.results[j.result_slot];  // written field by field, from LDS
         r.info = info;
         // searchSystem.cpp:695-711
         r.classification = info.status == 2u ? 2u : (info.numberOfSimulationTicksSurvived > j.M + 1u ? 1u : 0u);
@@ -901,7 +1120,7 @@ __device__ void run_entry(const BatchArgs &a, const DevJob &j, uint32_t *lds, ui
         r.t_end = wall_clock64();
         r.err = ok ? 0u : 1u;
         r.pad_ = 0u;
-        a.results[j.result_slot] = r;
+        r.computed = sh.computed;
         if constexpr (SERVE) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
             __hip_atomic_store(a.done + j.result_slot, j.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -922,7 +1141,9 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a)
     while (true) {
         if (threadIdx.x == 0) sh.claim = atomicAdd(a.next_job, 1u);
         __syncthreads();
-        const uint32_t jid = (uint32_t)sh.claim;
+        // the same in every thread: a scalar register, so the job's fields
+        // load into scalar registers (the rule tables must: see apply_rule_consts)
+        const uint32_t jid = __builtin_amdgcn_readfirstlane((uint32_t)sh.claim);
         __syncthreads();
         if (jid >= a.njobs) break;
         const DevJob j = a.jobs[jid];

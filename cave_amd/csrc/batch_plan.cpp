@@ -117,7 +117,7 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
         s.spawn_mode = CAVE_SPAWN_RANDOM;
         s.density = 0.25;
         s.seed = i;
-        s.flags = CAVE_SIM_NO_EARLY_EXIT | (erd ? CAVE_SIM_SHAPE_ERD : 0u);
+        s.flags = CAVE_SIM_NO_EARLY_EXIT | CAVE_SIM_FULL_GRID | (erd ? CAVE_SIM_SHAPE_ERD : 0u);
     }
     BatchKnobs k;
     k.threads = key.threads;
@@ -511,6 +511,7 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
             tm.end_us = (double)(hres[i].t_end - t0) * us_per_tick;
             tm.blocks = plan.jobs[i].parts;
             tm.items_per_thread = plan.ipt[i];
+            tm.computed_cells = hres[i].computed;
         }
     }
     for (const DevJob &j : plan.jobs)

@@ -301,15 +301,14 @@ __device__ __forceinline__ void apply_rule_rows(const Bits<5> *n, uint32_t born_
         const uint32_t bm = 0u - ((born_lut >> v) & 1u);
         const uint32_t sm = 0u - ((surv_lut >> v) & 1u);
         // pol[b] = ~0 where bit b of the count is 1: a one-bit sign-extending
-        // field extract, one scalar op per bit.  The empty asm pins each mask
-        // to a scalar register: left to itself the compiler builds the top
-        // bit's mask as a vector select (one VALU op per count).
+        // field extract, one scalar op per bit.  readfirstlane keeps each mask
+        // in a scalar register (the count is the same in every lane): left to
+        // itself the compiler builds the top bit's mask as a vector select
+        // (one VALU op per count).
         uint32_t pol[5];
 #pragma unroll
-        for (int b = 0; b < 5; b++) {
-            pol[b] = (uint32_t)__builtin_amdgcn_sbfe((int)v, b, 1);
-            asm volatile("" : "+s"(pol[b]));
-        }
+        for (int b = 0; b < 5; b++)
+            pol[b] = __builtin_amdgcn_readfirstlane((uint32_t)__builtin_amdgcn_sbfe((int)v, b, 1));
 #pragma unroll
         for (int r = 0; r < R; r++) {
             uint32_t mt = ~(n[r].b[0] ^ pol[0]);

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define CAVE_ABI_VERSION 1
+#define CAVE_ABI_VERSION 2
 
 /* ---- status codes ---- */
 enum {
@@ -251,6 +251,10 @@ typedef struct CaveSimSpec {
  * neighbours (C ignored), W wraps every axis.  Cube specs ignore W, as the
  * cube search kernel does (searchSimulationCube.slang:26-29). */
 #define CAVE_SIM_SHAPE_ERD 8u
+/* compute every cell on every tick.  By default the persistent engine
+ * computes only the cells within one cell of a cell that has been live so
+ * far (all others are 0 and stay 0); the results are the same either way. */
+#define CAVE_SIM_FULL_GRID 16u
 
 typedef struct CaveSimResult {
     CaveGridInfo info;       /* GridInfo at completion (manager semantics) */
@@ -283,6 +287,9 @@ typedef struct CaveSimTiming {
     double end_us;   /* device clock when its result was written */
     uint32_t blocks; /* blocks the simulation was split over */
     uint32_t items_per_thread;
+    /* cells its ticks computed, summed over ticks: the active regions' cells
+     * (all X*Y*Z per tick with CAVE_SIM_FULL_GRID) */
+    uint64_t computed_cells;
 } CaveSimTiming;
 
 int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *results,

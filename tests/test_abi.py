@@ -40,13 +40,13 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(cave_amd.SimSpec) == 72
     assert C.sizeof(cave_amd.SimResult) == 88
     assert C.sizeof(cave_amd.BatchOptions) == 40
-    assert cave_amd.TIMING_DTYPE.itemsize == 24
+    assert cave_amd.TIMING_DTYPE.itemsize == 32
     assert C.sizeof(cave_amd.ServeOptions) == 24
     assert C.sizeof(cave_amd.ServeCompletion) == 136
 
 
 def test_abi_version():
-    assert cave_amd.lib().cave_abi_version() == 1
+    assert cave_amd.lib().cave_abi_version() == 2
 
 
 def test_host_rule_strings_match_oracle():
