@@ -458,6 +458,7 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
 struct Region {
     int32_t lo[3], hi[3];
     uint32_t w0, nw, y0, ny, za, zb, tz, n;
+    uint32_t mnw, mny;  // ceil(2^32 / nw), ceil(2^32 / ny) (0 for 1): see divmod_magic
     uint64_t cells;
 };
 
@@ -490,22 +491,49 @@ __device__ __forceinline__ void plan_region(Region &r, const JobGeo &g)
     r.ny = (uint32_t)(r.hi[1] - r.lo[1] + 1);
     r.za = (uint32_t)za;
     r.zb = (uint32_t)zb;
+    // ceil(2^32 / b) = floor((2^32 - 1) / b) + 1
+    r.mnw = r.nw == 1u ? 0u : 0xFFFFFFFFu / r.nw + 1u;
+    r.mny = r.ny == 1u ? 0u : 0xFFFFFFFFu / r.ny + 1u;
     const uint32_t nzp = r.zb - r.za, cols = r.nw * r.ny;
     const uint32_t nzc = min(nzp, max(1u, div_ceil_small(g.nitems, cols)));
     r.tz = div_ceil_small(nzp, nzc);
     r.n = cols * div_ceil_small(nzp, r.tz);
 }
 
+// a / b and a % b with m = ceil(2^32 / b) (m = 0 for b = 1): the high word
+// of a * m is a / b + a * e / 2^32 with e < 1 floored, so for a < 2^26 it is
+// the quotient or one more, and one select corrects it.  A few instructions
+// instead of a ~35-instruction integer division per item.
+__device__ __forceinline__ uint32_t divmod_magic(uint32_t a, uint32_t b, uint32_t m, uint32_t &rem)
+{
+    uint32_t q = b == 1u ? a : __umulhi(a, m);
+    q -= q * b > a ? 1u : 0u;
+    rem = a - q * b;
+    return q;
+}
+
 // One tick over the region's items (plan_region).
-template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE>
+// FASTDIV: the items' index arithmetic by divmod_magic (kernels without split
+// jobs; in the others the two multipliers' scalar registers make the kernel
+// spill, and integer division is used).
+template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE, bool FASTDIV>
 __device__ __forceinline__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                            const RuleConsts &rc, const Region &r, StatAcc &acc)
 {
     for (uint32_t it = threadIdx.x; it < r.n; it += blockDim.x) {
-        const int w = (int)(r.w0 + it % r.nw);
-        const uint32_t rest = it / r.nw;
-        const int y = (int)(r.y0 + rest % r.ny);
-        const int z0 = (int)(r.za + (rest / r.ny) * r.tz);
+        uint32_t wr, yr, rest, zc;
+        if constexpr (FASTDIV) {
+            rest = divmod_magic(it, r.nw, r.mnw, wr);
+            zc = divmod_magic(rest, r.ny, r.mny, yr);
+        } else {
+            wr = it % r.nw;
+            rest = it / r.nw;
+            yr = rest % r.ny;
+            zc = rest / r.ny;
+        }
+        const int w = (int)(r.w0 + wr);
+        const int y = (int)(r.y0 + yr);
+        const int z0 = (int)(r.za + zc * r.tz);
         const int z1 = min(z0 + (int)r.tz, (int)r.zb);
         if constexpr (SHAPE != kBatchShapeCube)
             run_item_erd<FLAGS, COH, GLOBAL, SHAPE == kBatchShapeErdWrap>(g, S, acur, anext, rc, w, y, z0, z1, acc);
@@ -516,28 +544,28 @@ __device__ __forceinline__ void tick_items(const JobGeo &g, const Words<GLOBAL> 
 // flags: F|E|C of a cube job, F|E of an ERD job.  The shape is a template
 // parameter (a kernel instantiation per shape): with both shapes' tick code in
 // one kernel this compiler fails ("illegal VGPR to SGPR copy").
-template <bool COH, bool GLOBAL, uint32_t SHAPE>
+template <bool COH, bool GLOBAL, uint32_t SHAPE, bool FASTDIV>
 __device__ __forceinline__ void run_tick(uint32_t flags, const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                          const RuleConsts &rc, const Region &r, StatAcc &acc)
 {
     if constexpr (SHAPE != kBatchShapeCube) {
         switch (flags & 3u) {
-        case 1: tick_items<1, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
-        case 2: tick_items<2, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
-        case 3: tick_items<3, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
-        default: tick_items<0, COH, GLOBAL, SHAPE>(g, S, acur, anext, rc, r, acc); break;
+        case 1: tick_items<1, COH, GLOBAL, SHAPE, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+        case 2: tick_items<2, COH, GLOBAL, SHAPE, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+        case 3: tick_items<3, COH, GLOBAL, SHAPE, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+        default: tick_items<0, COH, GLOBAL, SHAPE, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
         }
         return;
     }
     switch (flags & 7u) {
-    case 1: tick_items<1, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 2: tick_items<2, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 3: tick_items<3, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 4: tick_items<4, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 5: tick_items<5, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 6: tick_items<6, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    case 7: tick_items<7, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
-    default: tick_items<0, COH, GLOBAL, kBatchShapeCube>(g, S, acur, anext, rc, r, acc); break;
+    case 1: tick_items<1, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 2: tick_items<2, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 3: tick_items<3, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 4: tick_items<4, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 5: tick_items<5, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 6: tick_items<6, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    case 7: tick_items<7, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
+    default: tick_items<0, COH, GLOBAL, kBatchShapeCube, FASTDIV>(g, S, acur, anext, rc, r, acc); break;
     }
 }
 
@@ -1001,10 +1029,15 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         reg.zb = __builtin_amdgcn_readfirstlane(sh.reg.zb);
         reg.tz = __builtin_amdgcn_readfirstlane(sh.reg.tz);
         reg.n = __builtin_amdgcn_readfirstlane(sh.reg.n);
-        if (threadIdx.x == 0) sh.computed += sh.reg.cells;
-        if (j.tier == kTierLds) run_tick<false, false, SHAPE>(j.flags, g, SL, acur, anext, rc, reg, acc);
-        else if (multi) run_tick<true, true, SHAPE>(j.flags, g, SG, acur, anext, rc, reg, acc);
-        else run_tick<false, true, SHAPE>(j.flags, g, SG, acur, anext, rc, reg, acc);
+        if constexpr (!FAST_SPLIT) {
+            reg.mnw = __builtin_amdgcn_readfirstlane(sh.reg.mnw);
+            reg.mny = __builtin_amdgcn_readfirstlane(sh.reg.mny);
+        }
+        uint64_t tick_cells = 0u;  // thread 0: this tick's region cells, read before it plans the next
+        if (threadIdx.x == 0) tick_cells = sh.reg.cells;
+        if (j.tier == kTierLds) run_tick<false, false, SHAPE, !FAST_SPLIT>(j.flags, g, SL, acur, anext, rc, reg, acc);
+        else if (multi) run_tick<true, true, SHAPE, !FAST_SPLIT>(j.flags, g, SG, acur, anext, rc, reg, acc);
+        else run_tick<false, true, SHAPE, !FAST_SPLIT>(j.flags, g, SG, acur, anext, rc, reg, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
         uint32_t tot[3];
@@ -1062,6 +1095,7 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
             last.changed = born + dying;
             const bool stop = (info.status == 1u && !(j.sim_flags & CAVE_SIM_NO_EARLY_EXIT)) || t + 1u >= j.ticks;
             sh.done = stop ? 1u : 0u;
+            sh.computed += tick_cells;
         }
         __syncthreads();
         if (threadIdx.x == 0 && !packed && (sh.done || a.per_tick)) {
@@ -1074,9 +1108,10 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         const uint32_t tmp = acur;
         acur = anext;
         anext = tmp;
-        const bool done = sh.done;
-        __syncthreads();
-        if (done) break;
+        // No barrier here: thread 0 next writes sh.done and sh.reg after the
+        // next tick's first barrier, which every thread reaches only after
+        // reading them, and the next tick's partials are read after it too.
+        if (sh.done) break;
     }
     // the next phase of the general rendezvous
     uint32_t phase = packed ? 1u : ticks_run + 1u;
