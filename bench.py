@@ -268,7 +268,7 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     ra = cv.Aggregate()
     ra.record_batch(specs, res)
     with tempfile.TemporaryDirectory() as d:
-        path = os.path.join(d, "cfg3_aggregate.jsonl")
+        path = os.path.join(d, "uniform_search_aggregate.jsonl")
         ra.write_jsonl(path)
         with open(path) as f:
             records = sum(1 for _ in f)

@@ -669,6 +669,46 @@ __device__ __forceinline__ uint32_t wave_pk_min(uint32_t v)
     });
 }
 
+// The same reductions over lanes 0..15 only (one DPP row: the per-wave
+// partials of a block of at most 16 waves): the quad and row rotation steps
+// of wave_reduce_dpp, without the two row broadcasts; read from lane 0.
+template <typename Op>
+__device__ __forceinline__ int32_t row16_reduce_dpp(int32_t v, int32_t ident, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    return __builtin_amdgcn_readlane(v, 0);
+}
+
+__device__ __forceinline__ uint32_t row16_pk_min(uint32_t v)
+{
+    return (uint32_t)row16_reduce_dpp((int32_t)v, -1, [](int32_t x, int32_t y) {
+        return (int32_t)pk_min_u16((uint32_t)x, (uint32_t)y);
+    });
+}
+
+// Three sums and three packed minima over lanes 0..15, step by step across
+// the six values, so that no DPP read waits on the instruction just before it.
+__device__ __forceinline__ void row16_sums_pk_mins(uint32_t s[3], uint32_t m[3])
+{
+#define CAVE_ROW16_STEP(ctl)                                                                                          \
+    for (int k = 0; k < 3; k++) {                                                                                     \
+        s[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s[k], ctl, 0xF, 0xF, false);                           \
+        m[k] = pk_min_u16(m[k], (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m[k], ctl, 0xF, 0xF, false));        \
+    }
+    CAVE_ROW16_STEP(0xB1)   // quad_perm [1,0,3,2]
+    CAVE_ROW16_STEP(0x4E)   // quad_perm [2,3,0,1]
+    CAVE_ROW16_STEP(0x124)  // row_ror:4
+    CAVE_ROW16_STEP(0x128)  // row_ror:8
+#undef CAVE_ROW16_STEP
+    for (int k = 0; k < 3; k++) {
+        s[k] = (uint32_t)__builtin_amdgcn_readlane((int)s[k], 0);
+        m[k] = (uint32_t)__builtin_amdgcn_readlane((int)m[k], 0);
+    }
+}
+
 // Block min / max of the bounding box; wave 0 (thread 0 among it) receives
 // it.  scratch: 3 words per wave.
 __device__ __forceinline__ void block_bbox(const StatAcc &acc, uint32_t *scratch, int32_t bmin[3], int32_t bmax[3])
@@ -680,10 +720,14 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, uint32_t *scratch
     if (lane == 0)
         for (int k = 0; k < 3; k++) scratch[3 * wave + k] = w[k];
     __syncthreads();
-    if (wave == 0) {  // lane i reduces wave i's partials
+    if (wave == 0) {  // lane i < 16 reduces wave i's partials (nw <= 16)
+        // every lane loads (lanes past nw read a stale slot and drop it), so
+        // the three loads are issued together
         const bool has = lane < nw;
-        const uint32_t *p = scratch + 3 * (has ? lane : 0);
-        for (int k = 0; k < 3; k++) w[k] = wave_pk_min(has ? p[k] : 0xFFFFFFFFu);
+        const uint32_t *p = scratch + 3 * (lane & 15);
+        uint32_t pv[3];
+        for (int k = 0; k < 3; k++) pv[k] = p[k];
+        for (int k = 0; k < 3; k++) w[k] = row16_pk_min(has ? pv[k] : 0xFFFFFFFFu);
         box_unpack(w, bmin, bmax);
     }
 }
@@ -706,14 +750,21 @@ __device__ __forceinline__ void block_stats(const StatAcc &acc, uint32_t *ssum, 
         for (int k = 0; k < 3; k++) sbox[3 * wave + k] = w[k];
     }
     __syncthreads();
-    if (wave == 0) {  // lane i reduces wave i's partials (nw <= 16)
+    if (wave == 0) {  // lane i < 16 reduces wave i's partials (nw <= 16)
+        // every lane loads (lanes past nw read a stale slot and drop it), so
+        // the six loads are issued together
         const bool has = lane < nw;
-        const uint32_t *q = ssum + 3 * (has ? lane : 0);
-        const uint32_t *p = sbox + 3 * (has ? lane : 0);
+        const uint32_t *q = ssum + 3 * (lane & 15);
+        const uint32_t *p = sbox + 3 * (lane & 15);
         for (int k = 0; k < 3; k++) {
-            tot[k] = wave_sum_u32(has ? q[k] : 0u);
-            w[k] = wave_pk_min(has ? p[k] : 0xFFFFFFFFu);
+            tot[k] = q[k];
+            w[k] = p[k];
         }
+        for (int k = 0; k < 3; k++) {
+            tot[k] = has ? tot[k] : 0u;
+            w[k] = has ? w[k] : 0xFFFFFFFFu;
+        }
+        row16_sums_pk_mins(tot, w);
         box_unpack(w, bmin, bmax);
     }
 }

@@ -124,7 +124,7 @@ def test_relaunch_after_idle_close():
         assert key(got[i].result) == key(r), f"request {i}"
 
 
-def test_cfg5_envelope_against_the_cpu():
+def test_serving_envelope_edge_192_500_ticks_against_the_cpu():
     """The serving workload's own envelope: the 24 costliest requests of a 2 s
     stream of it (edges 176..191, F/FE/FEV, C in 2..16, 374..498-tick budgets,
     about half of them alive to the end), served together so the large ones
