@@ -86,8 +86,9 @@ struct StatAcc {
     int32_t bmax[3] = {-1, -1, -1};
 };
 
-// Reduces a StatAcc over the block; thread 0 returns the block total in *out.
-// scratch must hold 10 * (blockDim.x / 64) words of LDS.
+// Reduces a StatAcc over the block (at most 1024 threads); thread 0 returns
+// the block total in *out.  scratch must hold 10 * (blockDim.x / 64) words of
+// LDS.
 __device__ inline void block_reduce_stats(const StatAcc &a, StatPart *out, int32_t *scratch)
 {
     const int lane = threadIdx.x & 63;
@@ -103,19 +104,39 @@ __device__ inline void block_reduce_stats(const StatAcc &a, StatPart *out, int32
         w[4] = m0; w[5] = m1; w[6] = m2; w[7] = x0; w[8] = x1; w[9] = x2;
     }
     __syncthreads();
-    if (wave == 0) {  // lane w reduces wave w's partials (nwaves <= 64)
+    if (wave == 0) {  // lane w < 16 reduces wave w's partials (nwaves <= 16: one DPP row)
+        // Every lane loads (lanes past nwaves read wave 0's slot and drop it),
+        // so the ten loads issue together behind one wait; the ten reductions
+        // then take the four in-row DPP steps side by side.
         const bool has = lane < nwaves;
         const int32_t *v = scratch + (has ? lane : 0) * 10;
-        StatPart p;
-        p.dead = wave_sum_u32(has ? (uint32_t)v[0] : 0u);
-        p.dying = wave_sum_u32(has ? (uint32_t)v[1] : 0u);
-        p.born = wave_sum_u32(has ? (uint32_t)v[2] : 0u);
-        p.surv = wave_sum_u32(has ? (uint32_t)v[3] : 0u);
-        for (int k = 0; k < 3; k++) {
-            p.bmin[k] = wave_min_i32(has ? v[4 + k] : 0x7fffffff);
-            p.bmax[k] = wave_max_i32(has ? v[7 + k] : -1);
+        int32_t x[10];
+        for (int k = 0; k < 10; k++) x[k] = v[k];
+        for (int k = 0; k < 4; k++) x[k] = has ? x[k] : 0;
+        for (int k = 4; k < 7; k++) x[k] = has ? x[k] : 0x7fffffff;
+        for (int k = 7; k < 10; k++) x[k] = has ? x[k] : -1;
+#define CAVE_ROW16_STEP(ctl)                                                                              \
+        for (int k = 0; k < 4; k++)                                                                      \
+            x[k] = (int32_t)((uint32_t)x[k] + (uint32_t)__builtin_amdgcn_update_dpp(0, x[k], ctl, 0xF, 0xF, false)); \
+        for (int k = 4; k < 7; k++) x[k] = min(x[k], __builtin_amdgcn_update_dpp(0x7fffffff, x[k], ctl, 0xF, 0xF, false)); \
+        for (int k = 7; k < 10; k++) x[k] = max(x[k], __builtin_amdgcn_update_dpp(-1, x[k], ctl, 0xF, 0xF, false));
+        CAVE_ROW16_STEP(0xB1)   // quad_perm [1,0,3,2]
+        CAVE_ROW16_STEP(0x4E)   // quad_perm [2,3,0,1]
+        CAVE_ROW16_STEP(0x124)  // row_ror:4
+        CAVE_ROW16_STEP(0x128)  // row_ror:8
+#undef CAVE_ROW16_STEP
+        if (lane == 0) {
+            StatPart p;
+            p.dead = (uint32_t)x[0];
+            p.dying = (uint32_t)x[1];
+            p.born = (uint32_t)x[2];
+            p.surv = (uint32_t)x[3];
+            for (int k = 0; k < 3; k++) {
+                p.bmin[k] = x[4 + k];
+                p.bmax[k] = x[7 + k];
+            }
+            *out = p;
         }
-        if (lane == 0) *out = p;
     }
 }
 
