@@ -641,7 +641,7 @@ def main():
         ap.error("--step-dims other than 512,512,512 needs --no-cpu (the CPU parity check runs the 512^3 grid)")
     variants = {}
     if not a.no_variants:
-        # The 512^3 nibble-layout leg runs first: its 1000 warm-up ticks and
+        # The 512^3 nibble-layout leg runs first: its 2000 warm-up ticks and
         # 500 timed ones also bring the GPU's clocks to their steady state for
         # an in-cache 512^3 step.  On an idle GPU the 20 ticks right after 5
         # warm-up ticks run ~7% slower while the clocks ramp up; right after
@@ -649,7 +649,7 @@ def main():
         # for a while (20.9 vs 20.0 us per launch), so those legs come after
         # the headline.
         log("step variant: nibble layout at 512^3")
-        variants["step_nibble"] = run_step_variant((GRID, GRID, GRID), "stream_nibble", 500, 1000)
+        variants["step_nibble"] = run_step_variant((GRID, GRID, GRID), "stream_nibble", 500, 2000)
     log(f"single-grid step: {a.steps} ticks of {dims[0]}x{dims[1]}x{dims[2]} FEV")
     wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup, dims)
     cells = dims[0] * dims[1] * dims[2]
