@@ -38,8 +38,12 @@ struct CaveStep {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t *cur = nullptr, *nxt = nullptr;
     StatPart *partials = nullptr;   // [chunk ticks][nparts]
-    CaveTickStats *dstats = nullptr; // [chunk ticks]
-    CaveTickStats *hstats = nullptr; // [chunk ticks], pinned host staging for the per-tick copy
+    // [chunk ticks] of per-tick rows in pinned, host-coherent memory that the
+    // reduction kernel writes directly; it then sets *hflag (host-coherent),
+    // which the host polls instead of waiting for the stream
+    CaveTickStats *hstats = nullptr, *hstats_dev = nullptr;
+    uint32_t *hflag = nullptr, *hflag_dev = nullptr;
+    uint32_t *dcount = nullptr;  // device counter of the reduction's finished blocks
     uint32_t chunk = 0;
     // instantiated tick loops: one graph per (ticks, which buffer is current)
     struct TickGraph {
@@ -131,8 +135,9 @@ void release(CaveStep *h)
     if (h->cur) (void)hipFree(h->cur);
     if (h->nxt) (void)hipFree(h->nxt);
     if (h->partials) (void)hipFree(h->partials);
-    if (h->dstats) (void)hipFree(h->dstats);
+    if (h->dcount) (void)hipFree(h->dcount);
     if (h->hstats) (void)hipHostFree(h->hstats);
+    if (h->hflag) (void)hipHostFree(h->hflag);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -230,11 +235,18 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
     // time the first time a run length is seen.
     if (const char *env = std::getenv("CAVE_STEP_GRAPHS")) h->use_graphs = env[0] == '1';
     CK(hipMalloc(&h->partials, sizeof(StatPart) * (size_t)h->nparts * h->chunk));
-    CK(hipMalloc(&h->dstats, sizeof(CaveTickStats) * h->chunk));
-    // Per-tick stats go device -> pinned staging -> caller: a device-to-host
-    // copy straight into a caller's pageable buffer of this size costs the
-    // runtime milliseconds per call.
-    CK(hipHostMalloc((void **)&h->hstats, sizeof(CaveTickStats) * h->chunk, hipHostMallocDefault));
+    // Per-tick rows go device -> pinned staging -> caller: the reduction
+    // kernel writes them straight into host-coherent memory and raises a
+    // flag there, so a call ends without a device-to-host copy and without
+    // the stream synchronisation's wake-up latency (a copy straight into a
+    // caller's pageable buffer of this size costs the runtime milliseconds).
+    CK(hipMalloc(&h->dcount, sizeof(uint32_t)));
+    CK(hipMemsetAsync(h->dcount, 0, sizeof(uint32_t), h->stream));
+    CK(hipHostMalloc((void **)&h->hstats, sizeof(CaveTickStats) * h->chunk,
+                     hipHostMallocCoherent | hipHostMallocMapped));
+    CK(hipHostMalloc((void **)&h->hflag, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped));
+    CK(hipHostGetDevicePointer((void **)&h->hstats_dev, h->hstats, 0));
+    CK(hipHostGetDevicePointer((void **)&h->hflag_dev, h->hflag, 0));
     if (spec->spawn_mode == CAVE_SPAWN_BUFFER) {
         CK(hipStreamSynchronize(h->stream));
         rc = cave_step_upload(h, spec->initial_cells, dev_words(h));
@@ -273,12 +285,30 @@ int cave_step_run(CaveStep *h, uint32_t ticks, CaveTickStats *per_tick, double *
         }
         if (n & 1u) std::swap(h->cur, h->nxt);
         CAVE_HIP(hipEventRecord(h->ev1, h->stream));
-        int rc = launch_reduce_ticks(h->partials, h->nparts, n, h->dstats, h->stream);
-        if (rc) return rc;
-        if (per_tick)
-            CAVE_HIP(hipMemcpyAsync(h->hstats, h->dstats, sizeof(CaveTickStats) * n, hipMemcpyDeviceToHost, h->stream));
-        CAVE_HIP(hipStreamSynchronize(h->stream));
-        if (per_tick) std::memcpy(per_tick + done, h->hstats, sizeof(CaveTickStats) * n);
+        if (per_tick) {
+            __atomic_store_n(h->hflag, 0u, __ATOMIC_RELEASE);
+            int rc = launch_reduce_ticks(h->partials, h->nparts, n, h->hstats_dev, h->stream, h->dcount, h->hflag_dev);
+            if (rc) return rc;
+            // The rows are complete once the flag is up.  A failed launch
+            // never raises it: the stream query reports the error (or the
+            // stream's end without the flag) instead.
+            for (uint32_t spin = 1;; spin++) {
+                if (__atomic_load_n(h->hflag, __ATOMIC_ACQUIRE)) break;
+                if ((spin & 1023u) == 0u) {
+                    const hipError_t q = hipStreamQuery(h->stream);
+                    if (q == hipSuccess) {
+                        if (__atomic_load_n(h->hflag, __ATOMIC_ACQUIRE)) break;
+                        (void)hipMemset(h->dcount, 0, sizeof(uint32_t));  // for the next call
+                        return fail(CAVE_E_DEVICE, "the per-tick reduction ended without its completion flag");
+                    }
+                    if (q != hipErrorNotReady) CAVE_HIP(q);
+                }
+            }
+            std::memcpy(per_tick + done, h->hstats, sizeof(CaveTickStats) * n);
+            CAVE_HIP(hipEventSynchronize(h->ev1));
+        } else {
+            CAVE_HIP(hipStreamSynchronize(h->stream));
+        }
         float ms = 0.f;
         CAVE_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         total_ms += ms;

@@ -148,8 +148,14 @@ __global__ void __launch_bounds__(256) k_step_gather(const uint32_t *__restrict_
 // ---------------------------------------------------------------------------
 // per-tick reduction of block partials: one block per tick
 // ---------------------------------------------------------------------------
+// out may be host memory mapped into the device (the step handle's staging).
+// With `flag` set, every block counts itself done on `count` (device memory,
+// 0 on entry) after its row is written; the last block resets the count and,
+// once every row is visible to the host, writes 1 to the host-mapped flag,
+// which the host polls instead of waiting for the stream.
 __global__ void __launch_bounds__(256) k_reduce_ticks(const StatPart *__restrict__ partials, uint32_t nparts,
-                                                      CaveTickStats *__restrict__ out)
+                                                      CaveTickStats *__restrict__ out, uint32_t *count,
+                                                      uint32_t *flag)
 {
     __shared__ int32_t red[10 * 4];
     const StatPart *p = partials + (size_t)blockIdx.x * nparts;
@@ -179,6 +185,15 @@ __global__ void __launch_bounds__(256) k_reduce_ticks(const StatPart *__restrict
             s.bbox_max[k] = tot.bmax[k];
         }
         out[blockIdx.x] = s;
+        if (flag) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // the row reaches memory the host reads
+            const uint32_t prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev + 1u == gridDim.x) {
+                __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
 }
 
@@ -298,10 +313,10 @@ int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
 }
 
 int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
-                        hipStream_t st)
+                        hipStream_t st, uint32_t *count, uint32_t *flag)
 {
     if (ticks == 0) return CAVE_OK;
-    hipLaunchKernelGGL(k_reduce_ticks, dim3(ticks), dim3(256), 0, st, partials, nparts, out);
+    hipLaunchKernelGGL(k_reduce_ticks, dim3(ticks), dim3(256), 0, st, partials, nparts, out, count, flag);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

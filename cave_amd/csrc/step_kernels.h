@@ -47,8 +47,11 @@ int launch_init_erd(uint32_t *cells, uint32_t X, uint32_t Y, uint32_t Z, const D
 int launch_gather(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, StatPart *partials,
                   hipStream_t st, const uint32_t *done = nullptr, bool wrap = false);
 
+// One block per tick reduces that tick's block partials into out[tick].  With
+// count (device memory, 0) and flag (host-mapped) given, the last block to
+// finish sets *flag to 1 once every row is visible to the host.
 int launch_reduce_ticks(const StatPart *partials, uint32_t nparts, uint32_t ticks, CaveTickStats *out,
-                        hipStream_t st);
+                        hipStream_t st, uint32_t *count = nullptr, uint32_t *flag = nullptr);
 
 // ---- bit-sliced streaming kernel ----
 struct StreamPlan {
