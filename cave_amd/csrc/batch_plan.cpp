@@ -257,13 +257,32 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     std::vector<double> cost(n);
     std::vector<uint32_t> key_bucket(n);
     std::map<std::pair<uint32_t, uint32_t>, BucketWork> buckets;  // (edge bucket, flags)
+    // Lists hold long runs of one grid size and one bucket (the seeds of a
+    // config): per-size values and the bucket's entry are carried over from
+    // the previous spec instead of recomputed (map entries do not move).
+    uint32_t prev_dims[3] = {0u, 0u, 0u}, prev_bucket = 0u;
+    uint64_t prev_words = 0u;
+    std::pair<uint32_t, uint32_t> prev_key{~0u, ~0u};
+    BucketWork *prev_b = nullptr;
     for (size_t i = 0; i < n; i++) {
         DevJob &j = plan.jobs[i];
         j = make_job(specs[i], (uint32_t)i);
-        words[i] = job_words(j.X, j.Y, j.Z);
+        if (j.X != prev_dims[0] || j.Y != prev_dims[1] || j.Z != prev_dims[2]) {
+            prev_dims[0] = j.X;
+            prev_dims[1] = j.Y;
+            prev_dims[2] = j.Z;
+            prev_words = job_words(j.X, j.Y, j.Z);
+            prev_bucket = edge_bucket(j.X, j.Y, j.Z);
+        }
+        words[i] = prev_words;
         cost[i] = (double)j.X * j.Y * j.Z * j.ticks;
-        key_bucket[i] = edge_bucket(j.X, j.Y, j.Z);
-        BucketWork &b = buckets[{key_bucket[i], tune_kind(j, plan.shape)}];
+        key_bucket[i] = prev_bucket;
+        const std::pair<uint32_t, uint32_t> key{prev_bucket, tune_kind(j, plan.shape)};
+        if (key != prev_key) {
+            prev_key = key;
+            prev_b = &buckets[key];
+        }
+        BucketWork &b = *prev_b;
         if (b.work == 0)
             for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
         b.work += cost[i];
@@ -291,8 +310,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         return it == g_rates.end() ? 0.0 : it->second;
     };
     if (tune) {
-        hipStream_t cst;
-        CAVE_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
+        hipStream_t cst = nullptr;  // created on the first calibration this plan needs
         int rc = CAVE_OK;
         for (const auto &[bk, bw] : buckets)
             for (uint32_t T : cand)
@@ -300,6 +318,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
                     if (rc) break;
                     const TuneKey key{bk.first, bk.second, T, ipt};
                     if (rate_of(key.bucket, key.flags, T, ipt) > 0) continue;
+                    if (!cst) CAVE_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
                     double r = 0;
                     rc = calibrate(key, bw.dims, cst, r);
                     if (!rc) {
@@ -307,8 +326,10 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
                         g_rates[key] = r;
                     }
                 }
-        (void)hipStreamSynchronize(cst);
-        (void)hipStreamDestroy(cst);
+        if (cst) {
+            (void)hipStreamSynchronize(cst);
+            (void)hipStreamDestroy(cst);
+        }
         if (rc) return rc;
     }
     // a candidate replaces the default (the first one listed) only when it
@@ -365,13 +386,28 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     const uint32_t resident = cu_count() * per_cu;
 
     // ---- single-block time estimates, splitting, order ----
+    // items per thread and rate per bucket, looked up in the (locked) rate
+    // table once per bucket rather than once per simulation
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, double>> bucket_choice;
+    for (const auto &[bk, bw] : buckets) {
+        double r = 0;
+        const uint32_t ipt = best_ipt(bk.first, bk.second, threads, &r);
+        bucket_choice[bk] = {ipt, r};
+    }
     std::vector<double> t(n);
     std::vector<uint32_t> ipt_of(n, ipts[0]);
     double total = 0;
+    prev_key = {~0u, ~0u};
+    std::pair<uint32_t, double> choice{ipts[0], 0.0};
     for (size_t i = 0; i < n; i++) {
         const DevJob &j = plan.jobs[i];
-        double r = 0;
-        ipt_of[i] = best_ipt(key_bucket[i], tune_kind(j, plan.shape), threads, &r);
+        const std::pair<uint32_t, uint32_t> key{key_bucket[i], tune_kind(j, plan.shape)};
+        if (key != prev_key) {
+            prev_key = key;
+            choice = bucket_choice[key];
+        }
+        ipt_of[i] = choice.first;
+        const double r = choice.second;
         // a bucket's device-wide rate is shared by `resident` blocks
         t[i] = r > 0 ? cost[i] * resident / r : cost[i];
         total += t[i];
@@ -381,6 +417,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     max_parts = std::min(max_parts, std::max(1u, resident / 4u));
     std::vector<std::tuple<int, double, size_t>> order(n);  // (unsplit, per-part time, job)
     double longest_part = 0;
+    uint32_t zc_key[5] = {0u, 0u, 0u, 0u, 0u}, zc_prev = 0u;  // part_zchunks of the previous (X, Y, Z, ipt, parts)
     for (size_t i = 0; i < n; i++) {
         DevJob &j = plan.jobs[i];
         uint32_t parts = 1;
@@ -390,7 +427,15 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             parts = std::max(1u, parts);
         }
         j.parts = parts;
-        j.zchunks = part_zchunks(j.X, j.Y, j.Z, threads, ipt_of[i], parts);
+        if (j.X != zc_key[0] || j.Y != zc_key[1] || j.Z != zc_key[2] || ipt_of[i] != zc_key[3] || parts != zc_key[4]) {
+            zc_key[0] = j.X;
+            zc_key[1] = j.Y;
+            zc_key[2] = j.Z;
+            zc_key[3] = ipt_of[i];
+            zc_key[4] = parts;
+            zc_prev = part_zchunks(j.X, j.Y, j.Z, threads, ipt_of[i], parts);
+        }
+        j.zchunks = zc_prev;
         plan.ipt[i] = ipt_of[i];
         if (parts > 1) {
             j.tier = kTierOwnRegion;
@@ -403,10 +448,13 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         order[i] = {parts > 1 ? 0 : 1, t[i] / parts, i};
         longest_part = std::max(longest_part, t[i] / parts);
     }
-    std::stable_sort(order.begin(), order.end(), [](const auto &x, const auto &y) {
+    const auto claim_order = [](const auto &x, const auto &y) {
         if (std::get<0>(x) != std::get<0>(y)) return std::get<0>(x) < std::get<0>(y);
         return std::get<1>(x) > std::get<1>(y);
-    });
+    };
+    // a list already in claim order (e.g. configs of one size and budget) is kept as is
+    if (!std::is_sorted(order.begin(), order.end(), claim_order))
+        std::stable_sort(order.begin(), order.end(), claim_order);
     plan.queue.reserve(n);
     for (const auto &o : order) {
         const DevJob &j = plan.jobs[std::get<2>(o)];
@@ -442,13 +490,19 @@ int run_plan(const BatchPlan &plan, size_t n, CaveSimResult *results, CaveBatchO
         CAVE_HIP(hipMemsetAsync(dsync.p, 0, sizeof(JobSync) * plan.nsync, st));
     }
     if (plan.final_words && (rc = dev_alloc(dfinal, sizeof(uint32_t) * plan.final_words))) return rc;
-    // word offsets -> device addresses
-    std::vector<DevJob> queue = plan.queue;
-    for (DevJob &e : queue) {
-        if (e.tier == kTierOwnRegion) e.region = (uint64_t)(dregion.as<uint32_t>() + e.region);
-        if (e.has_final) e.final_addr = (uint64_t)(dfinal.as<uint32_t>() + e.final_addr);
+    // word offsets -> device addresses, in a copy of the queue only when some
+    // entry holds one
+    std::vector<DevJob> queue;
+    const DevJob *qsrc = plan.queue.data();
+    if (plan.region_words || plan.final_words) {
+        queue = plan.queue;
+        for (DevJob &e : queue) {
+            if (e.tier == kTierOwnRegion) e.region = (uint64_t)(dregion.as<uint32_t>() + e.region);
+            if (e.has_final) e.final_addr = (uint64_t)(dfinal.as<uint32_t>() + e.final_addr);
+        }
+        qsrc = queue.data();
     }
-    CAVE_HIP(hipMemcpyAsync(djobs.p, queue.data(), sizeof(DevJob) * nq, hipMemcpyHostToDevice, st));
+    CAVE_HIP(hipMemcpyAsync(djobs.p, qsrc, sizeof(DevJob) * nq, hipMemcpyHostToDevice, st));
     CAVE_HIP(hipMemsetAsync(dnext.p, 0, sizeof(uint32_t), st));
 
     BatchArgs a{};
