@@ -237,3 +237,29 @@ def test_graph_replays_match_eager_and_oracle(kernel):
                 ref, st = O.step(ref, X, Y, Z, w0, w1)
                 assert rows[t].as_tuple() == st.as_tuple() == erows[t].as_tuple()
             assert np.array_equal(h.cells(), ref) and np.array_equal(eager.cells(), ref)
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: KNAME[k])
+def test_rows_after_runs_without_rows_and_across_chunks(kernel):
+    """Per-tick rows come back through host-mapped memory and a completion
+    flag; runs without rows skip the reduction.  Interleaved runs with and
+    without rows, and a run longer than one chunk of ticks (1024 on a grid
+    this small: two reductions, two flags in one call), against the oracle
+    tick by tick (searchSimulationCube.slang:78-247)."""
+    X, Y, Z, M, flags = 21, 9, 11, 3, 3
+    rng = cv.Rng(3)  # a rule whose cells still change on every tick past 1,000
+    birth, surv = rng.rule(18, 0.3), rng.rule(18, 0.4)
+    w0, w1 = ((M - 1) << 60) | birth, (flags << 60) | surv
+    packed = O.init_cells(X, Y, Z, M, (X, Y, Z), 0, 0.3, 5)
+    print(f"dims={X}x{Y}x{Z} M={M} kernel={KNAME[kernel]}", flush=True)
+    ref = packed
+    with cv.Step((X, Y, Z), M, flags, birth, surv, initial_cells=packed, kernel=kernel) as h:
+        for ticks, want_rows in ((3, False), (4, True), (2, False), (1, True), (1100, True), (5, False), (3, True)):
+            rows, _ = h.run(ticks, stats=want_rows)
+            for t in range(ticks):
+                ref, st = O.step(ref, X, Y, Z, w0, w1)
+                if want_rows:
+                    assert rows[t].as_tuple() == st.as_tuple(), f"run of {ticks}, tick {t}"
+            assert st.as_tuple()[5] > 0  # still changing
+            assert rows is None or len(rows) == ticks
+            assert np.array_equal(h.cells(), ref), f"cells after a run of {ticks}"
