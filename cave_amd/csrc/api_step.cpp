@@ -7,7 +7,8 @@
 // kernel's (searchSimulationCube.slang:78-247), which the render path's
 // (rayMarchCompute.slang:150-178) equals on non-wrapping grids; with the W
 // flag every axis wraps (rayMarchCompute.slang:80-108,
-// simulationRenderer.cpp:595) and the wrapping gather kernel runs.  With
+// simulationRenderer.cpp:595) on the planar streaming kernel (grids under
+// 1 GiB per buffer) or the gather kernel.  With
 // CAVE_STEP_GRAPHS=1 a run of ticks is one HIP graph of step launches
 // (captured once per chunk length and buffer parity, replayed after).
 #include <hip/hip_runtime.h>
@@ -93,7 +94,7 @@ int launch_ticks(CaveStep *h, uint32_t n, uint32_t *cur, uint32_t *nxt)
         int rc = h->kernel == CAVE_KERNEL_ERD
                      ? launch_erd(cur, nxt, h->g.X, h->g.Y, h->g.Z, h->rule, h->wrap, parts, h->stream)
                  : h->kernel != CAVE_KERNEL_GATHER
-                     ? launch_stream(cur, nxt, h->g, h->rule, h->plan, parts, h->stream)
+                     ? launch_stream(cur, nxt, h->g, h->rule, h->plan, parts, h->stream, nullptr, h->wrap)
                      : launch_gather(cur, nxt, h->g, h->rule, parts, h->stream, nullptr, h->wrap);
         if (rc) return rc;
         std::swap(cur, nxt);
@@ -186,11 +187,17 @@ int cave_step_create(const CaveStepSpec *spec, CaveStep **out)
 
     GridGeom sg = h->g;
     sg.pitch = 4u * ((h->g.X + 31u) / 32u);
-    const bool stream_ok = stream_supported(sg) && !h->wrap && h->shape == 0u;
+    // W: the planar streaming kernel wraps grids of under 1 GiB per buffer;
+    // the nibble layout and larger grids wrap on the gather kernel
+    GridGeom wg = sg;
+    wg.planar = 1u;
+    const bool wrap_ok = stream_wrap_supported(wg) && spec->kernel != CAVE_KERNEL_STREAM_NIBBLE;
+    const bool stream_ok = stream_supported(sg) && h->shape == 0u && (!h->wrap || wrap_ok);
     const bool want_stream = spec->kernel == CAVE_KERNEL_STREAM || spec->kernel == CAVE_KERNEL_STREAM_NIBBLE;
     if (want_stream && !stream_ok) {
         release(h);
-        return fail(CAVE_E_UNSUPPORTED, h->wrap ? "wrapping grids run on the gather kernel"
+        return fail(CAVE_E_UNSUPPORTED, h->wrap ? "wrapping grids run on the planar streaming kernel (under 1 GiB "
+                                                  "per buffer) or the gather kernel"
                                                 : "the streaming kernel needs X <= 2048");
     }
     if (h->shape == 1u) h->kernel = CAVE_KERNEL_ERD;

@@ -65,9 +65,11 @@ struct StreamPlan {
 };
 
 bool stream_supported(const GridGeom &g);
+// wrap (the W flag): the planar layout, grids of under 1 GiB per buffer
+bool stream_wrap_supported(const GridGeom &g);
 StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t waves);
 int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, const StreamPlan &p,
-                  StatPart *partials, hipStream_t st, const uint32_t *done = nullptr);
+                  StatPart *partials, hipStream_t st, const uint32_t *done = nullptr, bool wrap = false);
 
 // ---- plain per-config manager: reduces one tick's partials and applies the
 // search manager on the device (searchManager.slang:98-149) ----

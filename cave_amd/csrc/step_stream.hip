@@ -118,7 +118,13 @@ struct Carry {
 
 // XMODE: one of kXRows .. kXWave64.  NR rows are loaded per step: the RPL
 // owned rows and NH halo rows.
-template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE>
+// WRAP (the W flag, planar layout): every axis wraps (rayMarchCompute.slang:
+// 80-108, one term per offset).  Rows load y mod Y, so the rows past Y of a
+// wave's last tile and the halo rows are the wrapped rows (only rows < Y are
+// stored and counted); planes load z mod Z through one descriptor over the
+// whole grid; the first and last group of an x-row take their outer
+// neighbour cell from the row's other end.
+template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE, bool WRAP = false>
 struct StreamLane {
     static constexpr int NH = XMODE == kXWave64 ? 2 : 1;
     static constexpr int NR = RPL + NH;
@@ -145,16 +151,25 @@ struct StreamLane {
         return make_uint4(v[0], v[1], v[2], v[3]);
     }
 
+    // the plane a load of plane z reads: z itself (-1 and Z fall outside the
+    // slab and read zeros), or z mod Z with WRAP (z is in -1..Z)
+    __device__ __forceinline__ int plane_of(int z) const
+    {
+        if constexpr (WRAP) return z < 0 ? z + (int)a.Z : (z >= (int)a.Z ? z - (int)a.Z : z);
+        return z;
+    }
+
     __device__ __forceinline__ uint4 load(int r, int z) const
     {   // z = -1 wraps to a huge offset and z = Z lands at the slab end: both read zeros
-        return load_at(r, (uint32_t)(z - zbase) * plane_bytes);
+        return load_at(r, (uint32_t)(plane_of(z) - zbase) * plane_bytes);
     }
 
     __device__ __forceinline__ void store(int r, int z, const uint32_t P[4]) const
-    {
+    {   // WRAP: a row past Y holds a wrapped copy and is never stored
         typedef unsigned int v4u __attribute__((ext_vector_type(4)));
         const v4u v = {P[0], P[1], P[2], P[3]};
-        __builtin_amdgcn_raw_buffer_store_b128(v, rout, (int)(roff[r] + (uint32_t)(z - zbase) * plane_bytes), 0, 0);
+        const uint32_t ro = (WRAP && !yok[r]) ? kOutOfRange : roff[r];
+        __builtin_amdgcn_raw_buffer_store_b128(v, rout, (int)(ro + (uint32_t)(z - zbase) * plane_bytes), 0, 0);
     }
 
     __device__ __forceinline__ void planes_of(const uint4 &w, uint32_t P[4]) const
@@ -181,7 +196,7 @@ struct StreamLane {
         // Unconditional: past the walk's last plane the load gets an offset
         // the range check rejects (no memory access, zeros, never consumed).
         // A conditional load would be waited on at once.
-        const uint32_t zoff = more ? (uint32_t)(zi + dz - zbase) * plane_bytes : kNoPlane;
+        const uint32_t zoff = more ? (uint32_t)(plane_of(zi + dz) - zbase) * plane_bytes : kNoPlane;
 #pragma unroll
         for (int r = 0; r < NR; r++) co.w[r] = load_at(r, zoff);
 #pragma unroll
@@ -209,8 +224,19 @@ struct StreamLane {
                 prev = xg == 0u ? 0u : prev;
                 next = xg == lpr - 1u ? 0u : next;
             }
+            uint32_t wrap_r = 0u;
+            if constexpr (WRAP) {
+                // cell X-1 (bit b of group G-1) is the left neighbour of cell 0,
+                // cell 0 the right neighbour of cell X-1; lanes past G hold 0
+                const int base = (int)(lane - xg);
+                const uint32_t firstA = (uint32_t)__shfl((int)A, base, 64);
+                const uint32_t lastA = (uint32_t)__shfl((int)A, base + (int)a.G - 1, 64);
+                const uint32_t b = (a.X - 1u) & 31u;
+                if (xg == 0u) prev = ((lastA >> b) & 1u) << 31;
+                if (xg + 1u == a.G) wrap_r = (firstA & 1u) << b;
+            }
             const uint32_t Lm = PLANAR ? shift_left_neighbor_nat(A, prev) : shift_left_neighbor_il(A, prev);
-            const uint32_t Rm = PLANAR ? shift_right_neighbor_nat(A, next) : shift_right_neighbor_il(A, next);
+            const uint32_t Rm = (PLANAR ? shift_right_neighbor_nat(A, next) : shift_right_neighbor_il(A, next)) | wrap_r;
             rs[r] = row_sums(A, Lm, Rm);
         }
 
@@ -280,7 +306,9 @@ struct StreamLane {
             for (int r = 0; r < RPL; r++) {
                 const uint32_t valid = yok[r] ? vmask : 0u;
                 nrows += yok[r] ? 1u : 0u;
-                const uint32_t A = ci.A[r], NZ = ci.NZ[r];
+                // WRAP: a row past Y holds a wrapped copy, not zeros: its cells
+                // count as nothing
+                const uint32_t A = ci.A[r], NZ = WRAP ? ci.NZ[r] & valid : ci.NZ[r];
                 const uint32_t set_m = valid & ((A & ts[r]) | (~NZ & tb[r]));
                 const uint32_t sv = set_m & A;
                 surv += __popc(sv);
@@ -303,7 +331,7 @@ struct StreamLane {
 };
 
 
-template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE>
+template <uint32_t FLAGS, int RPL, bool PLANAR, int XMODE, bool WRAP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <= 2 ? 4 : 1))) k_step_stream(const uint4 *__restrict__ in, uint4 *__restrict__ out,
                                                      StreamArgs a, StatPart *__restrict__ partials,
                                                      const uint32_t *__restrict__ done)
@@ -323,7 +351,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
     StatAcc acc;
 
     if (gwave < a.total_waves) {
-        using Lane = StreamLane<FLAGS, RPL, PLANAR, XMODE>;
+        using Lane = StreamLane<FLAGS, RPL, PLANAR, XMODE, WRAP>;
         constexpr int NR = Lane::NR;
         const uint32_t ytile = gwave % a.ytiles;
         const uint32_t ztile = gwave / a.ytiles;
@@ -332,7 +360,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
         // the wave's slab: planes zbase .. zend-1 (its own planes and the halo
         // planes inside the grid), addressed through descriptors based there
         const uint32_t plane_bytes = a.G * a.Y * 16u;
-        const int zbase = max(z0 - 1, 0), zend = min(z1 + 1, (int)a.Z);
+        // WRAP: the wrapped halo planes lie anywhere in the grid, so the
+        // descriptors cover all of it (the plan keeps it below kMaxSlabBytes)
+        const int zbase = WRAP ? 0 : max(z0 - 1, 0), zend = WRAP ? (int)a.Z : min(z1 + 1, (int)a.Z);
         const size_t slab_at = (size_t)zbase * plane_bytes;
         const int slab_bytes = (int)((uint32_t)(zend - zbase) * plane_bytes);
         Lane L{__builtin_amdgcn_make_buffer_rsrc((void *)((const char *)in + slab_at), 0, slab_bytes, kBufferDword3),
@@ -356,7 +386,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPL <=
             const int y = r < RPL ? L.y_first + r
                                   : (Lane::NH == 2 ? (r == RPL ? L.y_first - 1 : L.y_first + RPL) : y_halo);
             L.yok[r] = gvalid && y >= 0 && y < (int)a.Y;
-            L.roff[r] = L.yok[r] ? ((uint32_t)y * a.G + L.xg) * 16u : kOutOfRange;
+            if constexpr (WRAP) {
+                // every row loads y mod Y (y is in -1 .. Y + rows of a wave);
+                // the halo row of a slot between two others is not used and
+                // loads nothing
+                const bool used = r < RPL || Lane::NH == 2 || L.slot == 0 || L.slot == L.nslots - 1;
+                const uint32_t yw = (uint32_t)((y % (int)a.Y + (int)a.Y) % (int)a.Y);
+                L.roff[r] = gvalid && used ? (yw * a.G + L.xg) * 16u : kOutOfRange;
+            } else {
+                L.roff[r] = L.yok[r] ? ((uint32_t)y * a.G + L.xg) * 16u : kOutOfRange;
+            }
             if (r < RPL) L.yany[r] = 0u;
         }
 
@@ -555,46 +594,53 @@ StreamPlan stream_plan(const GridGeom &g, uint32_t rpl, uint32_t tz, uint32_t wa
     return p;
 }
 
-template <uint32_t FLAGS, bool PLANAR>
+template <uint32_t FLAGS, bool PLANAR, bool WRAP>
 static void launch_flags(const uint4 *in, uint4 *out, const StreamArgs &a, const StreamPlan &p, StatPart *parts,
                          const uint32_t *done, hipStream_t st)
 {
     const dim3 grid(p.blocks), block(64 * p.waves);
     switch (p.rpl) {
-    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done); break;
+    case 1: hipLaunchKernelGGL((k_step_stream<FLAGS, 1, PLANAR, kXRows, WRAP>), grid, block, 0, st, in, out, a, parts, done); break;
     case 2:
         if (p.lpr == 64u)
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave64>), grid, block, 0, st, in, out, a, parts, done);
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave64, WRAP>), grid, block, 0, st, in, out, a, parts, done);
         else if (p.lpr == 32u)
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave32>), grid, block, 0, st, in, out, a, parts, done);
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXWave32, WRAP>), grid, block, 0, st, in, out, a, parts, done);
         else if (p.lpr == 16u)
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXFullRow>), grid, block, 0, st, in, out, a, parts, done);
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXFullRow, WRAP>), grid, block, 0, st, in, out, a, parts, done);
         else
-            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done);
+            hipLaunchKernelGGL((k_step_stream<FLAGS, 2, PLANAR, kXRows, WRAP>), grid, block, 0, st, in, out, a, parts, done);
         break;
-    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR, kXRows>), grid, block, 0, st, in, out, a, parts, done); break;
+    default: hipLaunchKernelGGL((k_step_stream<FLAGS, 4, PLANAR, kXRows, WRAP>), grid, block, 0, st, in, out, a, parts, done); break;
     }
 }
 
-template <bool PLANAR>
+template <bool PLANAR, bool WRAP>
 static void launch_layout(uint32_t flags, const uint4 *i4, uint4 *o4, const StreamArgs &a, const StreamPlan &p,
                           StatPart *partials, const uint32_t *done, hipStream_t st)
 {
     switch (flags & 7u) {
-    case 1: launch_flags<1, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 2: launch_flags<2, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 3: launch_flags<3, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 4: launch_flags<4, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 5: launch_flags<5, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 6: launch_flags<6, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    case 7: launch_flags<7, PLANAR>(i4, o4, a, p, partials, done, st); break;
-    default: launch_flags<0, PLANAR>(i4, o4, a, p, partials, done, st); break;  // no neighbourhood: counts stay 0
+    case 1: launch_flags<1, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 2: launch_flags<2, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 3: launch_flags<3, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 4: launch_flags<4, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 5: launch_flags<5, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 6: launch_flags<6, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    case 7: launch_flags<7, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;
+    default: launch_flags<0, PLANAR, WRAP>(i4, o4, a, p, partials, done, st); break;  // no neighbourhood: counts stay 0
     }
 }
 
+bool stream_wrap_supported(const GridGeom &g)
+{   // the planar layout, and one descriptor over the whole grid
+    return stream_supported(g) && g.planar && (uint64_t)g.pitch * 4u * g.Y * g.Z <= kMaxSlabBytes;
+}
+
 int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const DevRule &rule, const StreamPlan &p,
-                  StatPart *partials, hipStream_t st, const uint32_t *done)
+                  StatPart *partials, hipStream_t st, const uint32_t *done, bool wrap)
 {
+    if (wrap && !stream_wrap_supported(g))
+        return fail(CAVE_E_UNSUPPORTED, "the streaming kernel wraps planar grids of under 1 GiB only");
     if (p.rpl != 1 && p.rpl != 2 && p.rpl != 4) return fail(CAVE_E_INVALID, "rows per lane must be 1, 2 or 4");
     if (p.lpr > 16u && p.rpl != 2) return fail(CAVE_E_INVALID, "rows of more than 512 cells run 2 rows per lane");
     if ((uint64_t)(p.tz + 2u) * g.pitch * 4u * g.Y > kMaxSlabBytes)
@@ -617,8 +663,9 @@ int launch_stream(const uint32_t *in, uint32_t *out, const GridGeom &g, const De
     a.rc = rule_consts_for(rule.flags, rule.M, rule.born_lut, rule.surv_lut);
     const uint4 *i4 = reinterpret_cast<const uint4 *>(in);
     uint4 *o4 = reinterpret_cast<uint4 *>(out);
-    if (g.planar) launch_layout<true>(rule.flags, i4, o4, a, p, partials, done, st);
-    else launch_layout<false>(rule.flags, i4, o4, a, p, partials, done, st);
+    if (wrap) launch_layout<true, true>(rule.flags, i4, o4, a, p, partials, done, st);
+    else if (g.planar) launch_layout<true, false>(rule.flags, i4, o4, a, p, partials, done, st);
+    else launch_layout<false, false>(rule.flags, i4, o4, a, p, partials, done, st);
     CAVE_HIP(hipGetLastError());
     return CAVE_OK;
 }

@@ -157,7 +157,9 @@ typedef struct CaveStepSpec {
     uint32_t dims[3];
     CaveSimulationParameters params; /* raw-bitmask form (max-rule-bits 0).  The W flag wraps every
                                         axis, as the render path's step does (rayMarchCompute.slang:
-                                        84-102); such handles run the wrapping gather kernel */
+                                        84-102); such handles run the planar streaming kernel when
+                                        the grid's buffer is under 1 GiB (AUTO or STREAM), else the
+                                        gather kernel; STREAM_NIBBLE does not wrap */
     uint32_t spawn[3];               /* spawn box, centred, clamped to dims */
     uint32_t spawn_mode;             /* CAVE_SPAWN_* */
     double density;                  /* random spawn probability, evaluated in double */

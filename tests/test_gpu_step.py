@@ -95,13 +95,21 @@ WRAP_CASES = [
     # (X, Y, Z, M, flags): tiny tori count the same cell once per offset that reaches it
     (1, 1, 1, 1, 7), (2, 3, 1, 2, 3), (3, 2, 2, 4, 5), (7, 5, 3, 1, 7), (9, 8, 10, 3, 1),
     (33, 17, 9, 2, 3), (64, 3, 5, 15, 6), (100, 31, 7, 5, 7), (17, 1, 40, 1, 2),
+    # the streaming kernel's forms: several y tiles and z tiles, rows of 257..512
+    # cells (one DPP row), 513..1024 (half a wave), 1025..2048 (a wave), a
+    # whole last group, and rows per wave that do not divide Y
+    (33, 70, 9, 3, 7), (40, 9, 70, 2, 3), (300, 9, 5, 4, 7), (512, 5, 3, 7, 7), (600, 5, 4, 2, 3),
+    (1100, 3, 3, 3, 5), (2048, 2, 2, 1, 7),
 ]
+WRAP_KERNELS = [cv.KERNEL_AUTO, cv.KERNEL_GATHER]
 
 
+@pytest.mark.parametrize("kernel", WRAP_KERNELS, ids=["auto", "gather"])
 @pytest.mark.parametrize("X,Y,Z,M,flags", WRAP_CASES)
-def test_wrap_grids_state_by_state(X, Y, Z, M, flags):
+def test_wrap_grids_state_by_state(X, Y, Z, M, flags, kernel):
     """The W flag wraps every axis (the render path's step,
-    rayMarchCompute.slang:80-108): GPU vs oracle, state and counters per tick."""
+    rayMarchCompute.slang:80-108): GPU vs oracle, state and counters per tick,
+    on the planar streaming kernel (the default for W) and the gather kernel."""
     import np_model as NP
 
     rng = np.random.default_rng(X * 131 + Y * 17 + Z + M * 999 + flags)
@@ -112,9 +120,9 @@ def test_wrap_grids_state_by_state(X, Y, Z, M, flags):
     packed = NP.pack(dense)
     w0 = ((M - 1) & 0xF) << 60 | birth
     w1 = (flags | cv.WRAP) << 60 | surv
-    print(f"wrap case {X}x{Y}x{Z} M={M} flags={flags}", flush=True)
-    with cv.Step((X, Y, Z), M, flags | cv.WRAP, birth, surv, initial_cells=packed) as h:
-        assert h.kernel == cv.KERNEL_GATHER
+    print(f"wrap case {X}x{Y}x{Z} M={M} flags={flags} kernel={kernel}", flush=True)
+    with cv.Step((X, Y, Z), M, flags | cv.WRAP, birth, surv, initial_cells=packed, kernel=kernel) as h:
+        assert h.kernel == (cv.KERNEL_GATHER if kernel == cv.KERNEL_GATHER else cv.KERNEL_STREAM)
         ref = packed
         for t in range(6):
             ref, st_ref = O.step_render(ref, X, Y, Z, w0, w1)
@@ -123,9 +131,11 @@ def test_wrap_grids_state_by_state(X, Y, Z, M, flags):
             assert np.array_equal(h.cells(), ref), f"tick {t}"
 
 
-def test_wrap_needs_the_gather_kernel():
+def test_wrap_is_not_on_the_nibble_layout():
+    with cv.Step((8, 8, 8), 1, cv.FACES | cv.WRAP, 1, 1, kernel=cv.KERNEL_STREAM) as h:
+        assert h.kernel == cv.KERNEL_STREAM
     with pytest.raises(cv.CaveError) as e:
-        cv.Step((8, 8, 8), 1, cv.FACES | cv.WRAP, 1, 1, kernel=cv.KERNEL_STREAM)
+        cv.Step((8, 8, 8), 1, cv.FACES | cv.WRAP, 1, 1, kernel=cv.KERNEL_STREAM_NIBBLE)
     assert e.value.code == cv.CAVE_E_UNSUPPORTED
 
 
