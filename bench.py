@@ -73,7 +73,7 @@ def load_traffic(summary: str):
     return None
 
 
-def run_step_bench(steps: int, warmup: int, dims=(GRID, GRID, GRID)):
+def run_step_bench(steps: int, warmup: int, dims=(GRID, GRID, GRID), wrap: bool = False):
     """Times `steps` ticks after `warmup` untimed ones.  Returns the wall
     seconds and device ms of the timed ticks, the kernel name, the cells after
     all warmup + steps ticks and the last tick's statistics (for the parity
@@ -81,7 +81,7 @@ def run_step_bench(steps: int, warmup: int, dims=(GRID, GRID, GRID)):
     import cave_amd as cv
 
     birth, surv = workload_rule()
-    h = cv.Step(dims, 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv,
+    h = cv.Step(dims, 7, cv.FACES | cv.EDGES | cv.CORNERS | (cv.WRAP if wrap else 0), birth, surv,
                 spawn=dims, spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42)
     if warmup:
         h.run(warmup, stats=True)
@@ -97,14 +97,17 @@ def run_step_bench(steps: int, warmup: int, dims=(GRID, GRID, GRID)):
 def run_step_variant(dims, kernel_name: str, steps: int, warmup: int):
     """Event-timed launches of the step workload's rule on another grid or
     kernel form: the grid beyond the Infinity Cache (512 x 512 x 2048, a
-    512 MiB ping-pong pair against a 256 MB cache) and the nibble-layout form
-    of the streaming kernel at 512^3.  Returns the average launch time and
-    its algorithmic HBM rate (1 B per cell-update)."""
+    512 MiB ping-pong pair against a 256 MB cache), the nibble-layout form
+    of the streaming kernel at 512^3, and the wrapping step (the W flag: every
+    axis wraps, as in the render path's compute shader) at 512^3.  Returns the
+    average launch time and its algorithmic HBM rate (1 B per cell-update)."""
     import cave_amd as cv
 
     birth, surv = workload_rule()
-    kernel = {"stream": cv.KERNEL_STREAM, "stream_nibble": cv.KERNEL_STREAM_NIBBLE}[kernel_name]
-    with cv.Step(dims, 7, cv.FACES | cv.EDGES | cv.CORNERS, birth, surv, spawn=dims,
+    kernel = {"stream": cv.KERNEL_STREAM, "stream_nibble": cv.KERNEL_STREAM_NIBBLE,
+              "stream_wrap": cv.KERNEL_STREAM}[kernel_name]
+    flags = cv.FACES | cv.EDGES | cv.CORNERS | (cv.WRAP if kernel_name == "stream_wrap" else 0)
+    with cv.Step(dims, 7, flags, birth, surv, spawn=dims,
                  spawn_mode=cv.SPAWN_RANDOM, density=0.2, seed=42, kernel=kernel) as h:
         h.run(warmup, stats=False)
         t0 = time.perf_counter()
@@ -114,6 +117,7 @@ def run_step_variant(dims, kernel_name: str, steps: int, warmup: int):
     launch_s = dev_ms / 1e3 / steps
     gbs = cells / launch_s / 1e9
     summary = {((GRID, GRID, 4 * GRID), "stream"): "step_stream_512x512x2048_FEV_C8.md",
+               ((GRID, GRID, GRID), "stream_wrap"): "step_stream_wrap_512_FEV_C8.md",
                ((1024, 1024, 1024), "stream"): "step_stream_1024_FEV_C8.md"}.get((tuple(dims), kernel_name))
     return {"grid": list(dims), "kernel": f"step_{kernel_name}", "ticks": steps, "warmup": warmup,
             "traffic": load_traffic(summary) if summary else None,
@@ -571,6 +575,47 @@ def run_serving_bench(rate_hz=2000.0, duration_s=60.0, cpu=True, split_us=0, max
     return out
 
 
+def device_copy_rate(nbytes: int, copies: int = 100, warm: int = 20):
+    """Plain device-to-device copies of `nbytes` (hipMemcpyAsync on the null
+    stream, HIP events around `copies` of them after `warm` untimed ones):
+    the rate a kernel that reads and writes the same bytes once can expect.
+    Returns (us per copy, GB/s read + written)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the engine's library already loaded
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    src, dst, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)), "hipMalloc")
+        try:
+            ok(hip.hipMemset(src, 0x5A, C.c_size_t(nbytes)), "hipMemset")
+            ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+            ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+            copy = lambda: ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None), "hipMemcpyAsync")
+            for _ in range(warm):
+                copy()
+            ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+            for _ in range(copies):
+                copy()
+            ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+            ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+            ms = C.c_float()
+            ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+        finally:
+            hip.hipFree(dst)
+    finally:
+        hip.hipFree(src)
+    us = ms.value * 1e3 / copies
+    return us, 2 * nbytes / us / 1e3
+
+
 def copy_calibration(mib: int, copies: int):
     """K plain device-to-device copies of a MIB-MiB buffer: each reads and
     writes exactly MIB MiB, the yardstick for the HBM byte counters.  Also
@@ -608,6 +653,9 @@ def main():
     ap.add_argument("--step-dims", default=f"{GRID},{GRID},{GRID}",
                     help="grid of the single-grid step (X,Y,Z); other than 512^3 only with --no-cpu, "
                          "for profiling one size on its own")
+    ap.add_argument("--step-wrap", action="store_true",
+                    help="the single-grid step with the W flag (every axis wraps); only with --no-cpu, "
+                         "for profiling the wrapping step on its own")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the 512x512x2048 and nibble-layout step timings")
     ap.add_argument("--no-uneven", action="store_true")
@@ -639,6 +687,8 @@ def main():
     dims = tuple(int(v) for v in a.step_dims.split(","))
     if dims != (GRID, GRID, GRID) and not a.no_cpu:
         ap.error("--step-dims other than 512,512,512 needs --no-cpu (the CPU parity check runs the 512^3 grid)")
+    if a.step_wrap and not a.no_cpu:
+        ap.error("--step-wrap needs --no-cpu (the CPU parity check runs the non-wrapping workload)")
     variants = {}
     if not a.no_variants:
         # The 512^3 nibble-layout leg runs first: its 2000 warm-up ticks and
@@ -651,7 +701,7 @@ def main():
         log("step variant: nibble layout at 512^3")
         variants["step_nibble"] = run_step_variant((GRID, GRID, GRID), "stream_nibble", 500, 2000)
     log(f"single-grid step: {a.steps} ticks of {dims[0]}x{dims[1]}x{dims[2]} FEV")
-    wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup, dims)
+    wall, dev_ms, kernel, gpu_cells, gpu_last = run_step_bench(a.steps, a.warmup, dims, a.step_wrap)
     cells = dims[0] * dims[1] * dims[2]
     value = cells * a.steps / wall
     launch_s = dev_ms / 1e3 / a.steps
@@ -679,7 +729,14 @@ def main():
                      "kernel": f"step_{kernel}", "avg_launch_us": launch_s * 1e6},
         "device": {"name": dev, "cus": cus, "hbm_gib": round(mem / 2**30, 1)},
     }
+    # the yardstick beside the peak: a plain copy of one state buffer (cells / 2
+    # bytes read and as many written, the bytes of one tick) on the same GPU
+    copy_us, copy_gbs = device_copy_rate(cells // 2)
+    out["roofline"].update({"copy_gbs": copy_gbs, "copy_us": copy_us, "frac_of_copy": achieved / copy_gbs,
+                            "copy": f"hipMemcpyAsync of {cells // 2 >> 20} MiB device to device, 100 timed"})
     if not a.no_variants:
+        log("step variant: the wrapping step (W) at 512^3")
+        variants["step_wrap"] = run_step_variant((GRID, GRID, GRID), "stream_wrap", 500, 200)
         log("step variants: 512x512x2048 (beyond the Infinity Cache), large grids")
         variants["step_beyond_cache"] = run_step_variant((GRID, GRID, 4 * GRID), "stream", 200, 200)
         # the reference's large-grid test sizes (src/testing/testRunner.cpp:200-229) and 1024^3
