@@ -455,22 +455,29 @@ __device__ __forceinline__ void run_item_erd(const JobGeo &g, const Words<GLOBAL
 // y0 .. y0+ny-1, planes [za, zb) cut into z-chunks of tz planes, n items.
 // cells: the region's cells over the whole job (every part's planes, whole
 // words in x), for the computed-cells count.
-struct Region {
+struct Box {
     int32_t lo[3], hi[3];
-    uint32_t w0, nw, y0, ny, za, zb, tz, n;
-    uint32_t mnw, mny;  // ceil(2^32 / nw), ceil(2^32 / ny) (0 for 1): see divmod_magic
+};
+
+struct alignas(16) Region {
+    uint32_t w0, nw, y0, ny, za, zb, tz, n;  // every thread reads these eight words at a tick's start
+    Box box;
     uint64_t cells;
 };
 
 // ceil(a / b), b >= 1, from a float quotient corrected to the exact value:
-// a few instructions instead of the integer division's ~35 on the one thread
-// that plans the next tick.  For the item and plane counts it sees (a < 2^26)
-// the estimate is within one of the result, so each loop runs at most once.
+// a few instructions, and no branch, instead of the integer division's ~35 on
+// the one thread that plans the next tick.  For a < 2^22 the float product is
+// within 0.75 / b of a / b (v_rcp_f32 is within 1 ulp), so its ceiling is the
+// result, one less or one more, and two selects correct it; q * b stays near
+// a, below 2^32.  The planner's counts (items, planes, chunks) are far below
+// 2^22; larger ones take the integer division.
 __device__ __forceinline__ uint32_t div_ceil_small(uint32_t a, uint32_t b)
 {
+    if (a >= (1u << 22)) return a / b + (a % b != 0u ? 1u : 0u);
     uint32_t q = (uint32_t)__builtin_ceilf((float)a * __builtin_amdgcn_rcpf((float)b));
-    while (q * b < a) q++;
-    while (q > 0u && (q - 1u) * b >= a) q--;
+    q = q * b < a ? q + 1u : q;
+    q = (q > 0u && (q - 1u) * b >= a) ? q - 1u : q;
     return q;
 }
 
@@ -479,52 +486,53 @@ __device__ __forceinline__ uint32_t div_ceil_small(uint32_t a, uint32_t b)
 // gave the whole part (thread 0 computes it once per tick).
 __device__ __forceinline__ void plan_region(Region &r, const JobGeo &g)
 {
-    const int za = max(r.lo[2], (int)g.zlo), zb = min(r.hi[2] + 1, (int)g.zhi);
+    const Box &b = r.box;
+    const int za = max(b.lo[2], (int)g.zlo), zb = min(b.hi[2] + 1, (int)g.zhi);
     r.n = 0u;
     r.za = r.zb = (uint32_t)max(za, 0);
     r.w0 = r.nw = r.y0 = r.ny = 0u;
     r.tz = 1u;
-    if (za >= zb || r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1]) return;
-    r.w0 = (uint32_t)r.lo[0] >> 5;
-    r.nw = ((uint32_t)r.hi[0] >> 5) - r.w0 + 1u;
-    r.y0 = (uint32_t)r.lo[1];
-    r.ny = (uint32_t)(r.hi[1] - r.lo[1] + 1);
+    if (za >= zb || b.hi[0] < b.lo[0] || b.hi[1] < b.lo[1]) return;
+    r.w0 = (uint32_t)b.lo[0] >> 5;
+    r.nw = ((uint32_t)b.hi[0] >> 5) - r.w0 + 1u;
+    r.y0 = (uint32_t)b.lo[1];
+    r.ny = (uint32_t)(b.hi[1] - b.lo[1] + 1);
     r.za = (uint32_t)za;
     r.zb = (uint32_t)zb;
-    // ceil(2^32 / b) = floor((2^32 - 1) / b) + 1
-    r.mnw = r.nw == 1u ? 0u : 0xFFFFFFFFu / r.nw + 1u;
-    r.mny = r.ny == 1u ? 0u : 0xFFFFFFFFu / r.ny + 1u;
     const uint32_t nzp = r.zb - r.za, cols = r.nw * r.ny;
     const uint32_t nzc = min(nzp, max(1u, div_ceil_small(g.nitems, cols)));
     r.tz = div_ceil_small(nzp, nzc);
     r.n = cols * div_ceil_small(nzp, r.tz);
 }
 
-// a / b and a % b with m = ceil(2^32 / b) (m = 0 for b = 1): the high word
-// of a * m is a / b + a * e / 2^32 with e < 1 floored, so for a < 2^26 it is
-// the quotient or one more, and one select corrects it.  A few instructions
-// instead of a ~35-instruction integer division per item.
-__device__ __forceinline__ uint32_t divmod_magic(uint32_t a, uint32_t b, uint32_t m, uint32_t &rem)
+// a / b and a % b from rb = the float reciprocal of b (v_rcp_f32, 1 ulp).
+// For a < 2^22 the float product a * rb is within 0.75 / b of a / b, so its
+// truncation is the quotient, one less or one more, and one select corrects
+// it (b = 1 is exact: rcp(1) = 1).  A few instructions per item, and no
+// per-region constant for thread 0 to compute between ticks.
+__device__ __forceinline__ uint32_t divmod_rcp(uint32_t a, uint32_t b, float rb, uint32_t &rem)
 {
-    uint32_t q = b == 1u ? a : __umulhi(a, m);
-    q -= q * b > a ? 1u : 0u;
-    rem = a - q * b;
-    return q;
+    const uint32_t q = (uint32_t)((float)a * rb);
+    const int32_t r = (int32_t)(a - q * b), bi = (int32_t)b;
+    rem = (uint32_t)(r < 0 ? r + bi : (r >= bi ? r - bi : r));
+    return r < 0 ? q - 1u : (r >= bi ? q + 1u : q);
 }
 
 // One tick over the region's items (plan_region).
-// FASTDIV: the items' index arithmetic by divmod_magic (kernels without split
-// jobs; in the others the two multipliers' scalar registers make the kernel
-// spill, and integer division is used).
+// FASTDIV: the items' index arithmetic by float reciprocals (divmod_rcp) when
+// the region has fewer than 2^22 items (always, for grids the planner makes:
+// items ~ max(word columns x rows, threads x items per thread)); integer
+// division otherwise, and in kernels with split jobs.
 template <uint32_t FLAGS, bool COH, bool GLOBAL, uint32_t SHAPE, bool FASTDIV>
 __device__ __forceinline__ void tick_items(const JobGeo &g, const Words<GLOBAL> &S, uint32_t acur, uint32_t anext,
                            const RuleConsts &rc, const Region &r, StatAcc &acc)
 {
+    const bool fast = FASTDIV && r.n < (1u << 22);
     for (uint32_t it = threadIdx.x; it < r.n; it += blockDim.x) {
         uint32_t wr, yr, rest, zc;
-        if constexpr (FASTDIV) {
-            rest = divmod_magic(it, r.nw, r.mnw, wr);
-            zc = divmod_magic(rest, r.ny, r.mny, yr);
+        if (fast) {
+            rest = divmod_rcp(it, r.nw, __builtin_amdgcn_rcpf((float)r.nw), wr);
+            zc = divmod_rcp(rest, r.ny, __builtin_amdgcn_rcpf((float)r.ny), yr);
         } else {
             wr = it % r.nw;
             rest = it / r.nw;
@@ -771,39 +779,29 @@ __device__ __forceinline__ void block_stats(const StatAcc &acc, uint32_t *ssum, 
 
 // The region of the next tick (thread 0): the union u of the live boxes
 // known so far, grown by `grow` cells and clipped to the grid, or the whole
-// grid when `full`; re-planned only when it changed.  `fresh`: r holds no
-// region yet.
-__device__ __forceinline__ void set_region(Region &r, const int32_t ulo[3], const int32_t uhi[3], int grow,
-                                           const DevJob &j, const JobGeo &g, bool full, bool fresh)
+// grid when `full`; re-planned only when it changed from `cur` (the region's
+// box, loaded by the caller with u).  `fresh`: r holds no region yet.
+// Selects, not branches: this runs on one thread while the block waits.
+__device__ __forceinline__ void set_region(Region &r, const Box &u, const Box &cur, int grow, const DevJob &j,
+                                           const JobGeo &g, bool full, bool fresh)
 {
     const int32_t dim[3] = {(int32_t)j.X, (int32_t)j.Y, (int32_t)j.Z};
-    bool empty = false;
-    for (int k = 0; k < 3; k++) empty |= uhi[k] < ulo[k];
-    int32_t lo[3], hi[3];
+    const bool empty = (u.hi[0] < u.lo[0]) | (u.hi[1] < u.lo[1]) | (u.hi[2] < u.lo[2]);
+    Box b;
     bool same = !fresh;
     for (int k = 0; k < 3; k++) {
-        if (full) {
-            lo[k] = 0;
-            hi[k] = dim[k] - 1;
-        } else if (empty) {
-            lo[k] = 0;
-            hi[k] = -1;
-        } else {
-            lo[k] = max(0, ulo[k] - grow);
-            hi[k] = min(dim[k] - 1, uhi[k] + grow);
-        }
-        same = same && lo[k] == r.lo[k] && hi[k] == r.hi[k];
+        const int32_t lo = max(0, u.lo[k] - grow), hi = min(dim[k] - 1, u.hi[k] + grow);
+        b.lo[k] = full ? 0 : (empty ? 0 : lo);
+        b.hi[k] = full ? dim[k] - 1 : (empty ? -1 : hi);
+        same = same & (b.lo[k] == cur.lo[k]) & (b.hi[k] == cur.hi[k]);
     }
     if (same) return;
-    for (int k = 0; k < 3; k++) {
-        r.lo[k] = lo[k];
-        r.hi[k] = hi[k];
-    }
+    r.box = b;
     plan_region(r, g);
     r.cells = 0u;
     if (!empty || full) {
-        const uint32_t x0 = 32u * ((uint32_t)lo[0] >> 5), x1 = min(j.X, 32u * (((uint32_t)hi[0] >> 5) + 1u));
-        r.cells = (uint64_t)(x1 - x0) * (uint32_t)(hi[1] - lo[1] + 1) * (uint32_t)(hi[2] - lo[2] + 1);
+        const uint32_t x0 = 32u * ((uint32_t)b.lo[0] >> 5), x1 = min(j.X, 32u * (((uint32_t)b.hi[0] >> 5) + 1u));
+        r.cells = (uint64_t)(x1 - x0) * (uint32_t)(b.hi[1] - b.lo[1] + 1) * (uint32_t)(b.hi[2] - b.lo[2] + 1);
     }
 }
 
@@ -981,7 +979,7 @@ struct BlockShared {
     uint32_t sum[3 * 16];
     uint32_t box[3 * 16];
     Region reg;        // the cells of the next tick (thread 0 writes it between ticks)
-    int32_t ulo[3], uhi[3];  // the union of the live boxes so far (thread 0)
+    Box u;             // the union of the live boxes so far (thread 0)
     uint64_t computed;       // cells of the regions so far, whole job (thread 0)
     CaveGridInfo info;       // the job's manager state (thread 0)
     CaveTickStats last;      // its last tick's row (thread 0)
@@ -1047,11 +1045,13 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
     const bool full = SHAPE == kBatchShapeErdWrap || (j.sim_flags & CAVE_SIM_FULL_GRID) != 0u;
     if (threadIdx.x == 0) {  // U, kept in LDS between ticks
         const uint32_t dim[3] = {j.X, j.Y, j.Z};
+        Box u;
         for (int k = 0; k < 3; k++) {
-            sh.ulo[k] = (int32_t)j.init.box_lo[k];
-            sh.uhi[k] = (int32_t)min(j.init.box_hi[k], dim[k]) - 1;
+            u.lo[k] = (int32_t)j.init.box_lo[k];
+            u.hi[k] = (int32_t)min(j.init.box_hi[k], dim[k]) - 1;
         }
-        set_region(sh.reg, sh.ulo, sh.uhi, 1, j, g, full, true);
+        sh.u = u;
+        set_region(sh.reg, u, u, 1, j, g, full, true);
     }
     if (multi) ok = part_rendezvous<SERVE>(a, j, need, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
@@ -1071,18 +1071,18 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
     StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
         acc = StatAcc{};
-        Region reg;  // the same in every thread: scalar registers
-        reg.w0 = __builtin_amdgcn_readfirstlane(sh.reg.w0);
-        reg.nw = __builtin_amdgcn_readfirstlane(sh.reg.nw);
-        reg.y0 = __builtin_amdgcn_readfirstlane(sh.reg.y0);
-        reg.ny = __builtin_amdgcn_readfirstlane(sh.reg.ny);
-        reg.za = __builtin_amdgcn_readfirstlane(sh.reg.za);
-        reg.zb = __builtin_amdgcn_readfirstlane(sh.reg.zb);
-        reg.tz = __builtin_amdgcn_readfirstlane(sh.reg.tz);
-        reg.n = __builtin_amdgcn_readfirstlane(sh.reg.n);
-        if constexpr (!FAST_SPLIT) {
-            reg.mnw = __builtin_amdgcn_readfirstlane(sh.reg.mnw);
-            reg.mny = __builtin_amdgcn_readfirstlane(sh.reg.mny);
+        Region reg;  // the same in every thread: scalar registers, from two 16-byte LDS reads
+        {
+            const uint4 p0 = *reinterpret_cast<const uint4 *>(&sh.reg.w0);
+            const uint4 p1 = *reinterpret_cast<const uint4 *>(&sh.reg.za);
+            reg.w0 = __builtin_amdgcn_readfirstlane(p0.x);
+            reg.nw = __builtin_amdgcn_readfirstlane(p0.y);
+            reg.y0 = __builtin_amdgcn_readfirstlane(p0.z);
+            reg.ny = __builtin_amdgcn_readfirstlane(p0.w);
+            reg.za = __builtin_amdgcn_readfirstlane(p1.x);
+            reg.zb = __builtin_amdgcn_readfirstlane(p1.y);
+            reg.tz = __builtin_amdgcn_readfirstlane(p1.z);
+            reg.n = __builtin_amdgcn_readfirstlane(p1.w);
         }
         uint64_t tick_cells = 0u;  // thread 0: this tick's region cells, read before it plans the next
         if (threadIdx.x == 0) tick_cells = sh.reg.cells;
@@ -1104,22 +1104,28 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
                                              pmax, &sh.ok);
             if (!ok) break;
             if (threadIdx.x == 0) {  // the job's box of the previous tick
+                Box u = sh.u;
+                const Box cur = sh.reg.box;
                 for (int k = 0; k < 3; k++) {
-                    sh.ulo[k] = min(sh.ulo[k], pmin[k]);
-                    sh.uhi[k] = max(sh.uhi[k], pmax[k]);
+                    u.lo[k] = min(u.lo[k], pmin[k]);
+                    u.hi[k] = max(u.hi[k], pmax[k]);
                 }
-                set_region(sh.reg, sh.ulo, sh.uhi, 2, j, g, full, false);
+                sh.u = u;
+                set_region(sh.reg, u, cur, 2, j, g, full, false);
             }
         } else {
             // every tick: whether this tick is the last is known only after the totals
             if (multi) ok = part_rendezvous<SERVE>(a, j, need, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
             if (!ok) break;
             if (threadIdx.x == 0) {  // the job's box of this tick
+                Box u = sh.u;
+                const Box cur = sh.reg.box;
                 for (int k = 0; k < 3; k++) {
-                    sh.ulo[k] = min(sh.ulo[k], bmin[k]);
-                    sh.uhi[k] = max(sh.uhi[k], bmax[k]);
+                    u.lo[k] = min(u.lo[k], bmin[k]);
+                    u.hi[k] = max(u.hi[k], bmax[k]);
                 }
-                set_region(sh.reg, sh.ulo, sh.uhi, 1, j, g, full, false);
+                sh.u = u;
+                set_region(sh.reg, u, cur, 1, j, g, full, false);
             }
         }
         if (threadIdx.x == 0) {
