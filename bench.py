@@ -260,6 +260,11 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     specs = W.search_uniform(n_configs, seeds)
     cells = 49 ** 3
     cv.batch_run_np(specs[: seeds * 8])  # warm-up (code objects, allocator)
+    # The timed call is the second on the whole list: the first also grows
+    # the engine's buffers to the list's size (its wall time is reported too).
+    t0 = time.perf_counter()
+    cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
+    wall_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     res, dev_ms, tm, _ = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
     wall = time.perf_counter() - t0
@@ -287,7 +292,7 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
         "value_wall": n_configs / wall,
         "unit": "configs/s",
         "configs": n_configs, "seeds_per_config": seeds, "sims": len(specs),
-        "device_ms": dev_ms, "wall_ms": wall * 1e3,
+        "device_ms": dev_ms, "wall_ms": wall * 1e3, "wall_ms_first_call": wall_first * 1e3,
         "cell_updates": updates, "cell_updates_per_s": updates / (dev_ms / 1e3),
         "early_exit": "reference status-1 early death on",
         "classes": {"viable": int(agg["viable"].sum()), "possible": int(agg["possible"].sum()),

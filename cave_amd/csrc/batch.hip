@@ -989,6 +989,14 @@ struct BlockShared {
     DevJob job;  // k_serve: the claimed ring entry, copied from host memory
 };
 
+// What thread 0's manager reads between ticks, loaded before the statistics
+// barrier.
+struct MgrState {
+    Box u, cur;  // the union of the live boxes so far; the region's box
+    uint64_t computed, tick_cells;
+    uint32_t died, born, max_alive;
+};
+
 // Runs one queue entry -- one simulation, or one part of a split one -- with
 // the whole block: initial cells, the ticks with the search manager fused in
 // (searchManager.slang:98-149: totals, max / final alive, early death, tick
@@ -1052,6 +1060,7 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         }
         sh.u = u;
         set_region(sh.reg, u, u, 1, j, g, full, true);
+        sh.computed = 0u;
     }
     if (multi) ok = part_rendezvous<SERVE>(a, j, need, 0u, t3, b3, c3, t3, b3, c3, &sh.ok);
     else __syncthreads();
@@ -1066,7 +1075,6 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         packed = multi && !a.per_tick && !(j.sim_flags & CAVE_SIM_LEGACY_DEATH) && cells < (1u << 24) &&
                  j.parts < (1u << 16);
     uint32_t part_dying_all = 0u, part_dying_last = 0u;  // thread 0, packed jobs
-    if (threadIdx.x == 0) sh.computed = 0u;
     uint32_t ticks_run = 0;  // the same in every thread and every part
     StatAcc acc;
     for (uint32_t t = 0; ok && t < j.ticks; t++) {
@@ -1084,13 +1092,24 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
             reg.tz = __builtin_amdgcn_readfirstlane(p1.z);
             reg.n = __builtin_amdgcn_readfirstlane(p1.w);
         }
-        uint64_t tick_cells = 0u;  // thread 0: this tick's region cells, read before it plans the next
-        if (threadIdx.x == 0) tick_cells = sh.reg.cells;
         if (j.tier == kTierLds) run_tick<false, false, SHAPE, !FAST_SPLIT>(j.flags, g, SL, acur, anext, rc, reg, acc);
         else if (multi) run_tick<true, true, SHAPE, !FAST_SPLIT>(j.flags, g, SG, acur, anext, rc, reg, acc);
         else run_tick<false, true, SHAPE, !FAST_SPLIT>(j.flags, g, SG, acur, anext, rc, reg, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
+        // Thread 0 loads the manager's state now, while the block's slower
+        // waves finish, so its serial part after the statistics barrier waits
+        // on no LDS read.  Nothing writes these words during the tick.
+        // (Every thread loads them, as broadcast LDS reads: values set in
+        // thread 0 only, or zeroed elsewhere, cost the waves more.)
+        MgrState m;
+        m.u = sh.u;
+        m.cur = sh.reg.box;
+        m.tick_cells = sh.reg.cells;  // this tick's region, before thread 0 plans the next
+        m.computed = sh.computed;
+        m.died = info.totalNumberOfCellsThatDied;
+        m.born = info.totalNumberOfCellsThatWereBorn;
+        m.max_alive = info.maxAliveCount;
         uint32_t tot[3];
         int32_t bmin[3], bmax[3];
         block_stats(acc, sh.sum, sh.box, tot, bmin, bmax);
@@ -1104,30 +1123,27 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
                                              pmax, &sh.ok);
             if (!ok) break;
             if (threadIdx.x == 0) {  // the job's box of the previous tick
-                Box u = sh.u;
-                const Box cur = sh.reg.box;
                 for (int k = 0; k < 3; k++) {
-                    u.lo[k] = min(u.lo[k], pmin[k]);
-                    u.hi[k] = max(u.hi[k], pmax[k]);
+                    m.u.lo[k] = min(m.u.lo[k], pmin[k]);
+                    m.u.hi[k] = max(m.u.hi[k], pmax[k]);
                 }
-                sh.u = u;
-                set_region(sh.reg, u, cur, 2, j, g, full, false);
+                sh.u = m.u;
+                set_region(sh.reg, m.u, m.cur, 2, j, g, full, false);
             }
         } else {
             // every tick: whether this tick is the last is known only after the totals
             if (multi) ok = part_rendezvous<SERVE>(a, j, need, t + 1u, tot, bmin, bmax, tot, bmin, bmax, &sh.ok);
             if (!ok) break;
             if (threadIdx.x == 0) {  // the job's box of this tick
-                Box u = sh.u;
-                const Box cur = sh.reg.box;
                 for (int k = 0; k < 3; k++) {
-                    u.lo[k] = min(u.lo[k], bmin[k]);
-                    u.hi[k] = max(u.hi[k], bmax[k]);
+                    m.u.lo[k] = min(m.u.lo[k], bmin[k]);
+                    m.u.hi[k] = max(m.u.hi[k], bmax[k]);
                 }
-                sh.u = u;
-                set_region(sh.reg, u, cur, 1, j, g, full, false);
+                sh.u = m.u;
+                set_region(sh.reg, m.u, m.cur, 1, j, g, full, false);
             }
         }
+        bool stop = false;  // thread 0
         if (threadIdx.x == 0) {
             // packed jobs learn the dying total after the last tick: it is
             // taken out of the died total and the last row then (the u32
@@ -1135,27 +1151,25 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
             const uint32_t dying = packed ? 0u : tot[0], born = tot[1], surv = tot[2];
             const uint32_t dead = cells - dying - born - surv;
             // searchManager.slang:115-139
-            info.numberOfSimulationTicksSurvived++;
-            info.totalNumberOfCellsThatDied += dead;
-            info.totalNumberOfCellsThatWereBorn += born;
+            info.numberOfSimulationTicksSurvived = ticks_run;
+            info.totalNumberOfCellsThatDied = m.died + dead;
+            info.totalNumberOfCellsThatWereBorn = m.born + born;
             info.finalAliveCount = surv;
-            info.maxAliveCount = max(info.maxAliveCount, surv);
+            info.maxAliveCount = max(m.max_alive, surv);
             const bool died = (j.sim_flags & CAVE_SIM_LEGACY_DEATH) ? dead == cells : (surv == 0u && born == 0u);
-            if (died) info.status = 1u;
-            else if (info.numberOfSimulationTicksSurvived >= j.ticks) info.status = 2u;
-            else info.status = 0u;
+            info.status = died ? 1u : (ticks_run >= j.ticks ? 2u : 0u);
             last.snap.numberOfDeadCells = dead;
             last.snap.numberOfDyingCells = dying;
             last.snap.numberOfCellsBorn = born;
             last.snap.numberOfSurvivingCells = surv;
             last.live = born + surv;
             last.changed = born + dying;
-            const bool stop = (info.status == 1u && !(j.sim_flags & CAVE_SIM_NO_EARLY_EXIT)) || t + 1u >= j.ticks;
+            stop = (died && !(j.sim_flags & CAVE_SIM_NO_EARLY_EXIT)) || t + 1u >= j.ticks;
             sh.done = stop ? 1u : 0u;
-            sh.computed += tick_cells;
+            sh.computed = m.computed + m.tick_cells;
         }
         __syncthreads();
-        if (threadIdx.x == 0 && !packed && (sh.done || a.per_tick)) {
+        if (threadIdx.x == 0 && !packed && (stop || a.per_tick)) {
             for (int k = 0; k < 3; k++) {
                 last.bbox_min[k] = bmin[k];
                 last.bbox_max[k] = bmax[k];
@@ -1165,9 +1179,6 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         const uint32_t tmp = acur;
         acur = anext;
         anext = tmp;
-        // No barrier here: thread 0 next writes sh.done and sh.reg after the
-        // next tick's first barrier, which every thread reaches only after
-        // reading them, and the next tick's partials are read after it too.
         if (sh.done) break;
     }
     // the next phase of the general rendezvous
