@@ -670,11 +670,19 @@ __device__ __forceinline__ void box_unpack(const uint32_t w[3], int32_t bmin[3],
     }
 }
 
-__device__ __forceinline__ uint32_t wave_pk_min(uint32_t v)
+// The wave's box in the packed form: six minimum / maximum reductions, each
+// step one fused DPP instruction (v_min_i32_dpp), then packed from the
+// wave-uniform results with scalar instructions.  Reducing the three packed
+// words instead takes three instructions a step (v_pk_min_u16 takes no DPP
+// operand) plus the per-lane packing.
+__device__ __forceinline__ void wave_box(const StatAcc &acc, uint32_t w[3])
 {
-    return (uint32_t)wave_reduce_dpp((int32_t)v, -1, [](int32_t x, int32_t y) {
-        return (int32_t)pk_min_u16((uint32_t)x, (uint32_t)y);
-    });
+    int32_t mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+        mn[k] = wave_min_i32(acc.bmin[k]);
+        mx[k] = wave_max_i32(acc.bmax[k]);
+    }
+    box_pack(mn, mx, w);
 }
 
 // The same reductions over lanes 0..15 only (one DPP row: the per-wave
@@ -723,8 +731,7 @@ __device__ __forceinline__ void block_bbox(const StatAcc &acc, uint32_t *scratch
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     uint32_t w[3];
-    box_pack(acc.bmin, acc.bmax, w);
-    for (int k = 0; k < 3; k++) w[k] = wave_pk_min(w[k]);
+    wave_box(acc, w);
     if (lane == 0)
         for (int k = 0; k < 3; k++) scratch[3 * wave + k] = w[k];
     __syncthreads();
@@ -749,8 +756,7 @@ __device__ __forceinline__ void block_stats(const StatAcc &acc, uint32_t *ssum, 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     const uint32_t a = wave_sum_u32(acc.dying), b = wave_sum_u32(acc.born), c = wave_sum_u32(acc.surv);
     uint32_t w[3];
-    box_pack(acc.bmin, acc.bmax, w);
-    for (int k = 0; k < 3; k++) w[k] = wave_pk_min(w[k]);
+    wave_box(acc, w);
     if (lane == 0) {
         ssum[3 * wave] = a;
         ssum[3 * wave + 1] = b;
