@@ -88,10 +88,58 @@ bool tuner_enabled()
     return !(e && e[0] == '0');
 }
 
-// One calibration launch: every resident block runs a copy of a grid of the
-// bucket (all-counts rule, no early exit, a few ticks).
-int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double &rate)
+// Simulations in a calibration launch on the list's own specs: eight per
+// resident block of the smallest block size, so that every block size runs
+// the same simulations and each block several of them -- with one each, the
+// launch would time the longest-lived simulation, not the throughput.
+size_t sample_copies()
 {
+    return (size_t)cu_count() * (1024u / kThreadChoices[0]) * 8u;
+}
+
+// Largest calibration launch on the list's own specs, in cell-updates: below
+// it a bucket's rates are measured on its own simulations (49^3 x 20 ticks x
+// 8192 copies is 1.9e10, ~3 ms), above it on the synthetic grid.
+constexpr double kSampleBudget = 2.5e10;
+constexpr size_t kSampleMax = 1024;  // spec indices kept per bucket
+
+// One calibration launch.  With `sample` (indices into `specs` of the
+// bucket's simulations), every resident block runs one of them, cycling
+// through the sample: their rules, spawns, tick budgets and flags (final
+// hashes off), so the rate includes what the synthetic grid does not --
+// active regions, early deaths, per-tick fixed costs -- and a block size that
+// runs several of them per CU is seen at its worth.  The rate counts each
+// copy's budgeted ticks, as the planner's costs do.  Without a sample: every
+// resident block runs a copy of a grid of the bucket (all-counts rule, no
+// early exit, every cell, a few ticks).
+int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double &rate,
+              const CaveSimSpec *sample_specs = nullptr, const std::vector<uint32_t> *sample = nullptr)
+{
+    if (sample && !sample->empty()) {
+        // the same simulations for every block size and items per thread:
+        // the sample, repeated to sample_copies()
+        const size_t copies = std::max<size_t>(sample->size(), sample_copies());
+        std::vector<CaveSimSpec> specs(copies);
+        double work = 0;
+        for (size_t i = 0; i < copies; i++) {
+            specs[i] = sample_specs[(*sample)[i % sample->size()]];
+            specs[i].flags &= ~(uint32_t)CAVE_SIM_FINAL_HASH;
+            work += (double)specs[i].dims[0] * specs[i].dims[1] * specs[i].dims[2] * specs[i].ticks;
+        }
+        BatchKnobs k;
+        k.threads = key.threads;
+        k.ipt = key.ipt;
+        k.max_parts = 1;
+        k.tune = false;
+        BatchPlan plan;
+        int rc = plan_batch(specs.data(), copies, k, plan);
+        if (rc) return rc;
+        std::vector<CaveSimResult> res(copies);
+        CaveBatchOptions opt{};
+        if ((rc = run_plan(plan, copies, res.data(), &opt, st))) return rc;
+        rate = work / (std::max(opt.device_ms, 1e-3) * 1e-3);
+        return CAVE_OK;
+    }
     const uint32_t copies = cu_count() * std::max(1u, 1024u / key.threads);
     const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
     const bool erd = (key.flags & kTuneErd) != 0u;
@@ -137,6 +185,9 @@ int calibrate(const TuneKey &key, const uint32_t dims[3], hipStream_t st, double
 struct BucketWork {
     uint32_t dims[3];  // a representative grid of the bucket
     double work = 0;   // budgeted cell-updates
+    std::vector<uint32_t> sample;  // its first simulations (up to kSampleMax)
+    double sample_work = 0;        // their budgeted cell-updates
+    uint32_t rate_flags = 0;       // the rate table's flags for the bucket (kTuneSample when sampled)
 };
 
 }  // namespace
@@ -260,6 +311,10 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     // Lists hold long runs of one grid size and one bucket (the seeds of a
     // config): per-size values and the bucket's entry are carried over from
     // the previous spec instead of recomputed (map entries do not move).
+    // the tuner's bucket kind, with jobs that compute every cell apart
+    auto plan_kind = [&](const DevJob &j) {
+        return tune_kind(j, plan.shape) | ((j.sim_flags & CAVE_SIM_FULL_GRID) ? kTuneFull : 0u);
+    };
     uint32_t prev_dims[3] = {0u, 0u, 0u}, prev_bucket = 0u;
     uint64_t prev_words = 0u;
     std::pair<uint32_t, uint32_t> prev_key{~0u, ~0u};
@@ -277,7 +332,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         words[i] = prev_words;
         cost[i] = (double)j.X * j.Y * j.Z * j.ticks;
         key_bucket[i] = prev_bucket;
-        const std::pair<uint32_t, uint32_t> key{prev_bucket, tune_kind(j, plan.shape)};
+        const std::pair<uint32_t, uint32_t> key{prev_bucket, plan_kind(j)};
         if (key != prev_key) {
             prev_key = key;
             prev_b = &buckets[key];
@@ -286,6 +341,10 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         if (b.work == 0)
             for (int a = 0; a < 3; a++) b.dims[a] = specs[i].dims[a];
         b.work += cost[i];
+        if (b.sample.size() < kSampleMax) {
+            b.sample.push_back((uint32_t)i);
+            b.sample_work += cost[i];
+        }
         if (specs[i].flags & CAVE_SIM_FINAL_HASH) {
             j.has_final = 1u;
             j.final_addr = plan.final_words;  // word offset until run_plan places the staging
@@ -309,6 +368,15 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
         auto it = g_rates.find(TuneKey{bucket, flags, T, ipt});
         return it == g_rates.end() ? 0.0 : it->second;
     };
+    // A bucket is calibrated on its own simulations when a launch of them on
+    // every resident block of the smallest block size stays within
+    // kSampleBudget (the small grids, where per-tick fixed costs and active
+    // regions decide the block size), else on the synthetic grid.
+    for (auto &[bk, bw] : buckets) {
+        const size_t m = std::max<size_t>(1, bw.sample.size());
+        const double launch = bw.sample_work / (double)m * (double)std::max(m, sample_copies());
+        bw.rate_flags = bk.second | (launch <= kSampleBudget ? kTuneSample : 0u);
+    }
     if (tune) {
         hipStream_t cst = nullptr;  // created on the first calibration this plan needs
         int rc = CAVE_OK;
@@ -316,11 +384,12 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             for (uint32_t T : cand)
                 for (uint32_t ipt : ipts) {
                     if (rc) break;
-                    const TuneKey key{bk.first, bk.second, T, ipt};
+                    const TuneKey key{bk.first, bw.rate_flags, T, ipt};
                     if (rate_of(key.bucket, key.flags, T, ipt) > 0) continue;
                     if (!cst) CAVE_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
                     double r = 0;
-                    rc = calibrate(key, bw.dims, cst, r);
+                    rc = (bw.rate_flags & kTuneSample) ? calibrate(key, bw.dims, cst, r, specs, &bw.sample)
+                                                       : calibrate(key, bw.dims, cst, r);
                     if (!rc) {
                         std::lock_guard<std::mutex> g(g_tune_mu);
                         g_rates[key] = r;
@@ -353,7 +422,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
             double t = 0;
             for (const auto &[bk, bw] : buckets) {
                 double r = 0;
-                best_ipt(bk.first, bk.second, T, &r);
+                best_ipt(bk.first, bw.rate_flags, T, &r);
                 t += bw.work / std::max(r, 1.0);
             }
             return t;
@@ -391,7 +460,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, double>> bucket_choice;
     for (const auto &[bk, bw] : buckets) {
         double r = 0;
-        const uint32_t ipt = best_ipt(bk.first, bk.second, threads, &r);
+        const uint32_t ipt = best_ipt(bk.first, bw.rate_flags, threads, &r);
         bucket_choice[bk] = {ipt, r};
     }
     std::vector<double> t(n);
@@ -401,7 +470,7 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     std::pair<uint32_t, double> choice{ipts[0], 0.0};
     for (size_t i = 0; i < n; i++) {
         const DevJob &j = plan.jobs[i];
-        const std::pair<uint32_t, uint32_t> key{key_bucket[i], tune_kind(j, plan.shape)};
+        const std::pair<uint32_t, uint32_t> key{key_bucket[i], plan_kind(j)};
         if (key != prev_key) {
             prev_key = key;
             choice = bucket_choice[key];

@@ -35,6 +35,11 @@ struct BatchPlan {
 // The rate table's neighbourhood key: F|E|C for cube jobs; for the ERD
 // shape, whose ticks cost differently, kTuneErd | F|E, plus 4 with wrap.
 constexpr uint32_t kTuneErd = 16u;
+// Batch-plan keys only (the serve path's keys never carry them): jobs that
+// compute every cell (CAVE_SIM_FULL_GRID), and rates measured on a sample of
+// the list's own specs rather than on the synthetic grid.
+constexpr uint32_t kTuneFull = 32u;
+constexpr uint32_t kTuneSample = 64u;
 uint32_t tune_kind(const DevJob &j, uint32_t shape);
 // The persistent kernel form (kBatchShape*) a spec runs on.
 uint32_t spec_shape(const CaveSimSpec &s);

@@ -302,8 +302,11 @@ int cave_batch_run(const CaveSimSpec *specs, size_t n_specs, CaveSimResult *resu
  * cell_updates_per_s}: one calibration launch per (grid-edge bucket [0,32)
  * [32,48) [48,64) [64,96) [96,128) [128,..), neighbourhood, block size, items
  * per thread).  `flags` is F|E|C for cube grids; ERD grids, whose ticks cost
- * differently, are keyed 16 + F|E, plus 4 when they wrap.  No reference
- * counterpart. */
+ * differently, are keyed 16 + F|E, plus 4 when they wrap.  Batch lists add 32
+ * for simulations computing every cell (CAVE_SIM_FULL_GRID) and 64 when the
+ * rate was measured on a sample of the list's own simulations (buckets where
+ * eight of them per resident block stay within 2.5e10 cell-updates) rather
+ * than on the synthetic all-counts grid.  No reference counterpart. */
 int cave_batch_tuning_report(char *buf, size_t cap);
 
 /* ================================================================== */
