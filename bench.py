@@ -248,7 +248,7 @@ def active_region(specs, res, dev_ms, nominal, tm, label):
                     "computes them; computed_* count the cells computed"}
 
 
-def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
+def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True, block_threads=0):
     """Rule-search rate at 49^3 x 20 ticks (FE, C in 2..5, <= 2 + 2 rule bits,
     32 seeds per config).  Persistent batched engine over the whole list; the
     plain one-launch-per-config engine and the CPU oracle on bounded samples."""
@@ -263,10 +263,11 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
     # The timed call is the second on the whole list: the first also grows
     # the engine's buffers to the list's size (its wall time is reported too).
     t0 = time.perf_counter()
-    cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
+    cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, block_threads=block_threads, timing=True)
     wall_first = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res, dev_ms, tm, _ = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, timing=True)
+    res, dev_ms, tm, threads = cv.batch_run_np(specs, engine=cv.ENGINE_PERSISTENT, block_threads=block_threads,
+                                               timing=True)
     wall = time.perf_counter() - t0
     agg = W.aggregate_by_config(specs, res, seeds)
     # one aggregate JSONL record per rule config (RuleAggregator,
@@ -293,6 +294,7 @@ def run_search_bench(n_configs=4096, seeds=32, plain_configs=64, cpu=True):
         "unit": "configs/s",
         "configs": n_configs, "seeds_per_config": seeds, "sims": len(specs),
         "device_ms": dev_ms, "wall_ms": wall * 1e3, "wall_ms_first_call": wall_first * 1e3,
+        "block_threads": threads,
         "cell_updates": updates, "cell_updates_per_s": updates / (dev_ms / 1e3),
         "early_exit": "reference status-1 early death on",
         "classes": {"viable": int(agg["viable"].sum()), "possible": int(agg["possible"].sum()),
@@ -667,6 +669,8 @@ def main():
     ap.add_argument("--no-reference-sweep", action="store_true")
     ap.add_argument("--no-erd", action="store_true")
     ap.add_argument("--plain-configs", type=int, default=64)
+    ap.add_argument("--search-threads", type=int, default=0,
+                    help="block size of the uniform search launch (0: the tuner's choice); for profiling")
     ap.add_argument("--serve-split-us", type=int, default=0,
                     help="split target of the server (0 = its default)")
     ap.add_argument("--serve-max-parts", type=int, default=0,
@@ -757,7 +761,8 @@ def main():
     del gpu_cells
     if not a.no_search:
         log(f"rule search: {a.search_configs} configs x 32 seeds at 49^3")
-        out["search"] = run_search_bench(a.search_configs, plain_configs=a.plain_configs, cpu=not a.no_cpu)
+        out["search"] = run_search_bench(a.search_configs, plain_configs=a.plain_configs, cpu=not a.no_cpu,
+                                         block_threads=a.search_threads)
         if not a.no_uneven:
             log("rule search, uneven sizes: 2048 configs")
             out["search_uneven"] = run_uneven_bench(cpu=not a.no_cpu)
