@@ -223,6 +223,37 @@ def test_autotuned_uneven_batch():
         assert res["final_hash"][i] == O.canonical_hash(final, X, Y, Z)
 
 
+def test_list_calibration_and_block_sizes_agree():
+    # 49^3 search grids: the tuner measures this bucket on the list's own
+    # simulations (the report's flags carry 64), and every block size -- 256
+    # threads with the grids in the arena, 1024 with them in LDS -- gives the
+    # same results, which equal the oracle's
+    from cave_amd import workloads as W
+
+    specs = W.search_uniform(24, 8)
+    specs["sim_flags"] = cv.SIM_FINAL_HASH
+    print("tuned", flush=True)
+    res, _, _, threads = cv.batch_run_np(specs, timing=True)
+    assert threads in (256, 512, 1024)
+    assert any(e["flags"] & 64 for e in cv.tuning_report())
+    fields = ("died", "born", "ticks_survived", "status", "max_alive", "final_alive", "classification", "ticks_run",
+              "last", "final_hash")
+    for T in (256, 512, 1024):
+        print(f"threads {T}", flush=True)
+        got, _ = cv.batch_run_np(specs, block_threads=T)
+        for f in fields:
+            assert np.array_equal(got[f], res[f]), (T, f)
+    for i in range(0, len(specs), 37):
+        sp = specs[i]
+        X, Y, Z = (int(v) for v in sp["dims"])
+        init = O.init_cells(X, Y, Z, int(sp["M"]), tuple(int(v) for v in sp["spawn"]), int(sp["spawn_mode"]),
+                            float(sp["density"]), int(sp["seed"]))
+        t, info, _, final = O.run_rule(X, Y, Z, int(sp["M"]), int(sp["flags"]), int(sp["birth"]),
+                                       int(sp["survival"]), int(sp["ticks"]), init)
+        assert res["ticks_run"][i] == t and res["final_alive"][i] == info.finalAliveCount
+        assert res["final_hash"][i] == O.canonical_hash(final, X, Y, Z)
+
+
 @pytest.mark.parametrize("engine", [cv.ENGINE_PERSISTENT, cv.ENGINE_PLAIN])
 def test_empty_birth_or_survival_sets(engine):
     # empty sets are ordinary rules here (M = 1 with no birth count packs to
