@@ -1103,11 +1103,11 @@ __device__ __forceinline__ void run_entry(const BatchArgs &a, const DevJob &j, u
         else run_tick<false, true, SHAPE, !FAST_SPLIT>(j.flags, g, SG, acur, anext, rc, reg, acc);
         ticks_run++;
         acc.born -= acc.surv;  // run_item counted every cell set to M
-        // Thread 0 loads the manager's state now, while the block's slower
-        // waves finish, so its serial part after the statistics barrier waits
-        // on no LDS read.  Nothing writes these words during the tick.
-        // (Every thread loads them, as broadcast LDS reads: values set in
-        // thread 0 only, or zeroed elsewhere, cost the waves more.)
+        // The manager's inputs, loaded now, while the block's slower waves
+        // finish, so thread 0's serial part after the statistics barrier
+        // waits on no LDS read.  Nothing writes these words during the tick.
+        // Every thread loads them (broadcast LDS reads): loaded in thread 0
+        // only, or zeroed in the others, they cost the kernel more.
         MgrState m;
         m.u = sh.u;
         m.cur = sh.reg.box;
