@@ -380,21 +380,32 @@ int plan_batch(const CaveSimSpec *specs, size_t n, const BatchKnobs &k, BatchPla
     if (tune) {
         hipStream_t cst = nullptr;  // created on the first calibration this plan needs
         int rc = CAVE_OK;
-        for (const auto &[bk, bw] : buckets)
-            for (uint32_t T : cand)
-                for (uint32_t ipt : ipts) {
-                    if (rc) break;
-                    const TuneKey key{bk.first, bw.rate_flags, T, ipt};
-                    if (rate_of(key.bucket, key.flags, T, ipt) > 0) continue;
-                    if (!cst) CAVE_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
-                    double r = 0;
-                    rc = (bw.rate_flags & kTuneSample) ? calibrate(key, bw.dims, cst, r, specs, &bw.sample)
-                                                       : calibrate(key, bw.dims, cst, r);
-                    if (!rc) {
-                        std::lock_guard<std::mutex> g(g_tune_mu);
-                        g_rates[key] = r;
-                    }
-                }
+        auto measure = [&](uint32_t bucket, const BucketWork &bw, uint32_t T, uint32_t ipt) {
+            const TuneKey key{bucket, bw.rate_flags, T, ipt};
+            if (rc || rate_of(key.bucket, key.flags, T, ipt) > 0) return;
+            if (!cst && (rc = hipStreamCreateWithFlags(&cst, hipStreamNonBlocking) == hipSuccess ? CAVE_OK
+                                                                                                   : CAVE_E_DEVICE))
+                return;
+            double r = 0;
+            rc = (bw.rate_flags & kTuneSample) ? calibrate(key, bw.dims, cst, r, specs, &bw.sample)
+                                               : calibrate(key, bw.dims, cst, r);
+            if (!rc) {
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                g_rates[key] = r;
+            }
+        };
+        // every block size at the default items per thread, then the other
+        // items-per-thread choices at the block size that measured best
+        for (const auto &[bk, bw] : buckets) {
+            uint32_t bestT = cand.back();
+            double bestr = -1;
+            for (uint32_t T : cand) {
+                measure(bk.first, bw, T, ipts[0]);
+                const double r = rate_of(bk.first, bw.rate_flags, T, ipts[0]);
+                if (r > bestr) bestr = r, bestT = T;
+            }
+            for (size_t q = 1; q < ipts.size(); q++) measure(bk.first, bw, bestT, ipts[q]);
+        }
         if (cst) {
             (void)hipStreamSynchronize(cst);
             (void)hipStreamDestroy(cst);
